@@ -1,0 +1,511 @@
+// fx_api.cpp -- context management and the native C API (include/difxfx.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "difxfx.h"
+#include "fx_common.h"
+#include "fx_kernels.h"
+
+using namespace fx;
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(FX_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+enum { EV_D0, EV_D1, EV_F1, EV_C1, EV_X0, EV_X1, EV_N };
+
+struct fx_context {
+    FxPlan p;
+    int prec;
+    int G;              // FFTs per corner-turn group
+    int kgroups;        // groups per X-engine K step
+    int granule;        // G * kgroups: FFT slots written per K step
+    int nfg;            // groups held by the X input
+    int max_ffts;       // padded FFT capacity
+    long long cap_bytes, stride;
+    hipStream_t stream;
+    uint8_t* d_data;
+    double* d_model;    // delays [nant*3] then fileoffsets [nant]
+    int* d_iter;
+    FxDelay* d_dtab;
+    float2* d_tw;
+    float* d_subchf;
+    float* d_stepchf;
+    void* d_xin;
+    size_t xin_bytes;
+    float2* d_vacc;
+    size_t vacc_bytes;
+    int* d_counts;
+    int4* d_tasks;
+    int ntasks, waves;
+    float* d_scale;     // [nchan][Spad] (FP8)
+    float* d_scale_inv;
+    float2* d_cross;
+    float2* d_autos;
+    float2* d_tap;
+    int tap;
+    int last_nffts;     // entries in the delay table of the last fx_fstage
+    int timing;
+    hipEvent_t ev[EV_N];
+    double ms[4];
+    int nl[2];
+};
+
+static int xin_row_bytes(int prec)
+{
+    (void)prec;
+    return 16;  // F32: (re0, re1, im0, im1); F16: 4 x half2; F8: 8 x 2 fp8
+}
+
+extern "C" const char* fx_last_error(void) { return g_err; }
+
+extern "C" int fx_fft_granule(const fx_context* ctx) { return ctx ? ctx->granule : FX_EINVAL; }
+
+static void free_ctx(fx_context* c)
+{
+    void* bufs[] = {c->d_data, c->d_model, c->d_iter, c->d_dtab, c->d_tw, c->d_subchf, c->d_stepchf,
+                    c->d_xin, c->d_vacc, c->d_counts, c->d_tasks, c->d_scale, c->d_scale_inv,
+                    c->d_cross, c->d_autos, c->d_tap};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (int i = 0; i < EV_N; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" void fx_destroy(fx_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    free_ctx(ctx);
+}
+
+// Host-side plan: kernel.c:13-39, generalised to power-of-two nchan.
+static int make_plan(const fx_params* prm, FxPlan* p)
+{
+    memset(p, 0, sizeof(*p));
+    const int lg = fx_ilog2_exact(2 * prm->nchan);
+    if (lg < 0) return fail(FX_EINVAL, "nchan=%d is not a power of two", prm->nchan);
+    if (!fstage_supported(lg))
+        return fail(FX_EUNSUPPORTED, "FFT length %d not supported (512..8192)", 2 * prm->nchan);
+    if (prm->nant < 1) return fail(FX_EINVAL, "nant=%d", prm->nant);
+    if (!(prm->bw > 0)) return fail(FX_EINVAL, "bandwidth must be positive");
+    if (!(prm->lo >= 0 && prm->lo < 2147483647.0)) return fail(FX_EINVAL, "LO must be in [0, 2^31) Hz");
+    p->nant = prm->nant;
+    p->nchan = prm->nchan;
+    p->F = 2 * prm->nchan;
+    p->lg = lg;
+    p->stride = 1 << ((lg - 1) / 2);
+    p->sub = 2 * p->stride;
+    p->nstep = p->F / p->sub;
+    p->nfstep = p->nchan / p->stride;
+    p->S = 2 * prm->nant;
+    p->Spad = (p->S + FX_BLK - 1) / FX_BLK * FX_BLK;
+    p->nblk = p->Spad / FX_BLK;
+    p->nbp = p->nblk * (p->nblk + 1) / 2;
+    p->Ts = 1.0 / (2.0 * prm->bw);
+    p->lo = prm->lo;
+    p->bw = prm->bw;
+    p->lofrac = prm->lo - (double)(int)prm->lo;
+    p->fracLO = p->lofrac > 0.00000000001;
+    return FX_OK;
+}
+
+extern "C" int fx_create(const fx_params* prm, fx_context** out)
+{
+    if (!prm || !out) return fail(FX_EINVAL, "null argument");
+    *out = nullptr;
+    if (prm->prec < FX_PREC_F32 || prm->prec > FX_PREC_F8) return fail(FX_EINVAL, "bad precision %d", prm->prec);
+    if (prm->max_ffts < 1) return fail(FX_EINVAL, "max_ffts must be >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(FX_EHIP, "no HIP device available");
+    fx_context* c = new fx_context();
+    memset(c->ev, 0, sizeof(c->ev));
+    int rc = make_plan(prm, &c->p);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    FxPlan& p = c->p;
+    c->prec = prm->prec;
+    c->G = fx_fstage_group_size(prm->prec);
+    c->kgroups = prm->prec == FX_PREC_F16 ? 2 : prm->prec == FX_PREC_F8 ? 4 : 1;
+    c->granule = c->G * c->kgroups;
+    p.G = c->G;
+    c->max_ffts = (prm->max_ffts + c->granule - 1) / c->granule * c->granule;
+    c->nfg = c->max_ffts / c->G;
+    const long long need = (long long)p.nchan * (prm->max_ffts + 1); // allocDataHost.c:21
+    c->cap_bytes = prm->max_bytes_per_ant > need ? prm->max_bytes_per_ant : need;
+    c->stride = (c->cap_bytes + 64 + 255) / 256 * 256;
+
+#define ALLOC(ptr, bytes)                                                                   \
+    do {                                                                                    \
+        if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                             \
+            free_ctx(c);                                                                    \
+            return fail(FX_ENOMEM, "hipMalloc of %zu bytes failed", (size_t)(bytes));       \
+        }                                                                                   \
+    } while (0)
+
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_ctx(c);
+        return fail(FX_EHIP, "hipStreamCreate failed");
+    }
+    for (int i = 0; i < EV_N; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            free_ctx(c);
+            return fail(FX_EHIP, "hipEventCreate failed");
+        }
+    ALLOC(c->d_data, (size_t)c->stride * p.nant);
+    ALLOC(c->d_model, sizeof(double) * 4 * p.nant);
+    ALLOC(c->d_iter, sizeof(int) * c->max_ffts);
+    ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
+    ALLOC(c->d_tw, sizeof(float2) * p.F);
+    ALLOC(c->d_subchf, sizeof(float) * p.stride);
+    ALLOC(c->d_stepchf, sizeof(float) * p.nfstep);
+    c->xin_bytes = (size_t)p.nchan * c->nfg * p.Spad * xin_row_bytes(c->prec);
+    ALLOC(c->d_xin, c->xin_bytes);
+    c->vacc_bytes = (size_t)p.nchan * p.nbp * FX_BLK * FX_BLK * sizeof(float2);
+    ALLOC(c->d_vacc, c->vacc_bytes);
+    ALLOC(c->d_counts, sizeof(int) * p.nant * p.nant);
+    ALLOC(c->d_scale, sizeof(float) * (size_t)p.nchan * p.Spad);
+    ALLOC(c->d_scale_inv, sizeof(float) * (size_t)p.nchan * p.Spad);
+    const int nbl = p.nant * (p.nant - 1) / 2;
+    ALLOC(c->d_cross, sizeof(float2) * ((size_t)nbl > 0 ? (size_t)nbl : 1) * 4 * p.nchan);
+    ALLOC(c->d_autos, sizeof(float2) * (size_t)p.nant * 4 * p.nchan);
+
+    // twiddles and channel-frequency tables (FP64 evaluated, rounded to f32)
+    std::vector<float2> tw(p.F);
+    for (int m = 0; m < p.F; m++) {
+        const double ang = -2.0 * M_PI * (double)m / (double)p.F;
+        tw[m] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    std::vector<float> subchf(p.stride), stepchf(p.nfstep);
+    const double PI = 3.14159265358979323846;
+    for (int s = 0; s < p.stride; s++) subchf[s] = (float)((2 * PI * s * p.bw) / (double)p.nchan);     // kernel.c:164
+    for (int k = 0; k < p.nfstep; k++)
+        stepchf[k] = (float)((2 * PI * k * p.stride * p.bw) / (double)p.nchan);                      // kernel.c:165
+    // X-engine wave tasks: row block I, column blocks J0.. (up to 4)
+    std::vector<int4> tasks;
+    for (int I = 0; I < p.nblk; I++)
+        for (int J0 = I; J0 < p.nblk; J0 += 4) {
+            const int nb = (p.nblk - J0) < 4 ? (p.nblk - J0) : 4;
+            tasks.push_back(make_int4(I, J0, nb, fx_bp_index(I, J0, p.nblk)));
+        }
+    c->ntasks = (int)tasks.size();
+    c->waves = c->ntasks < 4 ? c->ntasks : 4;
+    ALLOC(c->d_tasks, sizeof(int4) * tasks.size());
+#undef ALLOC
+    // FP8 default scales: the spectrum of 2-bit noise has E|Y|^2 = F * E[L^2]
+    // with E[L^2] = 4.3128 for levels {+-1, +-3.3359} at +-0.98 sigma; map one
+    // standard deviation of each component to 16.
+    const float sigma = (float)sqrt(p.F * 4.3128 / 2.0);
+    std::vector<float> sc((size_t)p.nchan * p.Spad, sigma / 16.0f), sci((size_t)p.nchan * p.Spad, 16.0f / sigma);
+
+    if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * p.F, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_subchf, subchf.data(), sizeof(float) * p.stride, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_stepchf, stepchf.data(), sizeof(float) * p.nfstep, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_tasks, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_scale, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_scale_inv, sci.data(), sizeof(float) * sci.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->d_data, 0, (size_t)c->stride * p.nant) != hipSuccess ||
+        hipMemset(c->d_model, 0, sizeof(double) * 4 * p.nant) != hipSuccess ||
+        hipMemset(c->d_xin, 0, c->xin_bytes) != hipSuccess ||
+        hipMemset(c->d_vacc, 0, c->vacc_bytes) != hipSuccess ||
+        hipMemset(c->d_counts, 0, sizeof(int) * p.nant * p.nant) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        free_ctx(c);
+        return fail(FX_EHIP, "context initialisation failed");
+    }
+    *out = c;
+    return FX_OK;
+}
+
+extern "C" int fx_set_model(fx_context* c, const double* delays, const double* fileoffsets)
+{
+    if (!c || !delays || !fileoffsets) return fail(FX_EINVAL, "null argument");
+    std::vector<double> m(4 * (size_t)c->p.nant);
+    memcpy(m.data(), delays, sizeof(double) * 3 * c->p.nant);
+    memcpy(m.data() + 3 * c->p.nant, fileoffsets, sizeof(double) * c->p.nant);
+    HIPCHK(hipMemcpyAsync(c->d_model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_upload_data(fx_context* c, const uint8_t* const* data, long long nbytes)
+{
+    if (!c || !data) return fail(FX_EINVAL, "null argument");
+    if (nbytes < 0 || nbytes > c->cap_bytes)
+        return fail(FX_EINVAL, "nbytes=%lld exceeds capacity %lld", nbytes, c->cap_bytes);
+    for (int a = 0; a < c->p.nant; a++) {
+        if (!data[a]) return fail(FX_EINVAL, "null data for antenna %d", a);
+        HIPCHK(hipMemcpyAsync(c->d_data + (size_t)a * c->stride, data[a], (size_t)nbytes, hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_upload_data_strided(fx_context* c, const uint8_t* data, long long stride, long long nbytes)
+{
+    if (!c || !data) return fail(FX_EINVAL, "null argument");
+    if (nbytes < 0 || nbytes > c->cap_bytes || stride < nbytes)
+        return fail(FX_EINVAL, "nbytes=%lld stride=%lld capacity %lld", nbytes, stride, c->cap_bytes);
+    HIPCHK(hipMemcpy2DAsync(c->d_data, (size_t)c->stride, data, (size_t)stride, (size_t)nbytes, (size_t)c->p.nant,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_device_data(fx_context* c, void** dptr, long long* stride)
+{
+    if (!c || !dptr || !stride) return fail(FX_EINVAL, "null argument");
+    *dptr = c->d_data;
+    *stride = c->stride;
+    return FX_OK;
+}
+
+static void acc_time(fx_context* c, int slot, hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) == hipSuccess) c->ms[slot] += ms;
+}
+
+extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int split, int fft0, int* nslots)
+{
+    if (!c || !iteration) return fail(FX_EINVAL, "null argument");
+    if (numffts < 1 || split < 1) return fail(FX_EINVAL, "numffts=%d split=%d", numffts, split);
+    if (fft0 < 0 || fft0 % c->granule) return fail(FX_EINVAL, "fft0=%d not a multiple of %d", fft0, c->granule);
+    const int npad = (numffts + c->granule - 1) / c->granule * c->granule;
+    if (fft0 + npad > c->max_ffts)
+        return fail(FX_EINVAL, "FFT slots [%d,%d) exceed capacity %d", fft0, fft0 + npad, c->max_ffts);
+    const FxPlan& p = c->p;
+    const long long maxoffset = ((long long)numffts * split - 1) * p.F;
+    if (maxoffset > 0x7fffffffLL - 2LL * p.F) return fail(FX_EINVAL, "subint too long for 32-bit offsets");
+    // The kernel reads samples [offset, offset + F) with offset <= maxoffset.
+    const long long need = ((long long)numffts * split * p.F + 1) / 2;
+    if (need > c->cap_bytes)
+        return fail(FX_EINVAL, "input needs %lld bytes per antenna, capacity %lld", need, c->cap_bytes);
+    for (int k = 0; k < numffts; k++)
+        if (iteration[k] < 0 || (long long)iteration[k] * p.F > 0x7fffffffLL - p.F)
+            return fail(FX_EINVAL, "iteration[%d]=%d out of range", k, iteration[k]);
+    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+    if (c->tap && !c->d_tap) {
+        if (hipMalloc((void**)&c->d_tap, sizeof(float2) * (size_t)c->max_ffts * p.nant * 2 * p.nchan) != hipSuccess)
+            return fail(FX_ENOMEM, "tap allocation failed");
+    }
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_D0], c->stream));
+    HIPCHK(launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, numffts, npad, (int)maxoffset, c->d_dtab,
+                        c->stream));
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_D1], c->stream));
+    FStageArgs a;
+    a.p = p;
+    a.data = c->d_data;
+    a.data_stride = c->stride;
+    a.delays = c->d_dtab;
+    a.tw = c->d_tw;
+    a.subchf = c->d_subchf;
+    a.stepchf = c->d_stepchf;
+    a.out = c->d_xin;
+    a.nfg = c->nfg;
+    a.fg0 = fft0 / c->G;
+    a.ngroups = npad / c->G;
+    a.scale_inv = c->d_scale_inv;
+    a.tap = c->tap ? c->d_tap : nullptr;
+    HIPCHK(launch_fstage(a, c->prec, c->stream));
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_F1], c->stream));
+    HIPCHK(launch_counts(p.nant, npad, c->d_dtab, c->d_counts, c->stream));
+    if (c->timing) {
+        HIPCHK(hipEventRecord(c->ev[EV_C1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[EV_C1]));
+        acc_time(c, 0, c->ev[EV_D0], c->ev[EV_D1]);
+        acc_time(c, 1, c->ev[EV_D1], c->ev[EV_F1]);
+        acc_time(c, 2, c->ev[EV_F1], c->ev[EV_C1]);
+        c->nl[0]++;
+    }
+    c->last_nffts = npad;
+    if (nslots) *nslots = npad;
+    return FX_OK;
+}
+
+extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    if (fft0 < 0 || nslots < 0 || fft0 % c->granule || nslots % c->granule || fft0 + nslots > c->max_ffts)
+        return fail(FX_EINVAL, "bad slot range [%d,%d) (granule %d, capacity %d)", fft0, fft0 + nslots, c->granule,
+                    c->max_ffts);
+    if (nslots == 0) return FX_OK;
+    XArgs a;
+    a.p = c->p;
+    a.xin = c->d_xin;
+    a.nfg = c->nfg;
+    a.fg0 = fft0 / c->G;
+    a.ngroups = nslots / c->G;
+    a.vacc = c->d_vacc;
+    a.tasks = c->d_tasks;
+    a.ntasks = c->ntasks;
+    a.ngroups_wg = c->waves;
+    a.accumulate = 1;
+    a.scale = c->d_scale;
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_X0], c->stream));
+    HIPCHK(launch_xengine(a, c->prec, c->stream));
+    if (c->timing) {
+        HIPCHK(hipEventRecord(c->ev[EV_X1], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[EV_X1]));
+        acc_time(c, 3, c->ev[EV_X0], c->ev[EV_X1]);
+        c->nl[1]++;
+    }
+    return FX_OK;
+}
+
+extern "C" int fx_process(fx_context* c, int numffts, const int* iteration, int split)
+{
+    int ns = 0;
+    int rc = fx_fstage(c, numffts, iteration, split, 0, &ns);
+    if (rc) return rc;
+    return fx_xstage(c, 0, ns);
+}
+
+extern "C" int fx_reset(fx_context* c)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    HIPCHK(launch_zero(c->d_vacc, c->vacc_bytes, c->stream));
+    HIPCHK(launch_zero(c->d_counts, sizeof(int) * c->p.nant * c->p.nant, c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_read_vis(fx_context* c, float* cross, int* cross_counts, float* autos, int* auto_counts)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    const FxPlan& p = c->p;
+    const int nbl = p.nant * (p.nant - 1) / 2;
+    HIPCHK(launch_export(p, c->d_vacc, c->d_cross, c->d_autos, c->stream));
+    if (cross && nbl)
+        HIPCHK(hipMemcpyAsync(cross, c->d_cross, sizeof(float2) * (size_t)nbl * 4 * p.nchan, hipMemcpyDeviceToHost,
+                              c->stream));
+    if (autos)
+        HIPCHK(hipMemcpyAsync(autos, c->d_autos, sizeof(float2) * (size_t)p.nant * 4 * p.nchan,
+                              hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> cnt((size_t)p.nant * p.nant);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->d_counts, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (cross_counts) {
+        int b = 0;
+        for (int j = 0; j < p.nant; j++)
+            for (int k = j + 1; k < p.nant; k++) cross_counts[b++] = cnt[(size_t)j * p.nant + k];
+    }
+    if (auto_counts)
+        for (int j = 0; j < p.nant; j++) auto_counts[j] = cnt[(size_t)j * p.nant + j];
+    return FX_OK;
+}
+
+extern "C" int fx_read_delay_table(fx_context* c, int nffts, void* out)
+{
+    if (!c || !out) return fail(FX_EINVAL, "null argument");
+    if (nffts < 0 || nffts > c->max_ffts) return fail(FX_EINVAL, "nffts=%d", nffts);
+    HIPCHK(hipMemcpyAsync(out, c->d_dtab, sizeof(FxDelay) * (size_t)nffts * c->p.nant, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_enable_tap(fx_context* c, int enable)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    c->tap = enable ? 1 : 0;
+    return FX_OK;
+}
+
+extern "C" int fx_read_tap(fx_context* c, int nffts, float* out)
+{
+    if (!c || !out) return fail(FX_EINVAL, "null argument");
+    if (!c->d_tap) return fail(FX_EINVAL, "tap not enabled before fx_fstage");
+    if (nffts < 0 || nffts > c->max_ffts) return fail(FX_EINVAL, "nffts=%d", nffts);
+    HIPCHK(hipMemcpyAsync(out, c->d_tap, sizeof(float2) * (size_t)nffts * c->p.nant * 2 * c->p.nchan,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_set_fp8_scales(fx_context* c, const float* scales)
+{
+    if (!c || !scales) return fail(FX_EINVAL, "null argument");
+    const FxPlan& p = c->p;
+    std::vector<float> sc((size_t)p.nchan * p.Spad, 1.0f), sci((size_t)p.nchan * p.Spad, 1.0f);
+    for (int ch = 0; ch < p.nchan; ch++)
+        for (int s = 0; s < p.S; s++) {
+            const float v = scales[(size_t)ch * p.S + s];
+            if (!(v > 0.0f) || !std::isfinite(v)) return fail(FX_EINVAL, "scale[%d][%d] not positive", ch, s);
+            sc[(size_t)ch * p.Spad + s] = v;
+            sci[(size_t)ch * p.Spad + s] = 1.0f / v;
+        }
+    HIPCHK(hipMemcpyAsync(c->d_scale, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_scale_inv, sci.data(), sizeof(float) * sci.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_sync(fx_context* c)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_set_timing(fx_context* c, int enable)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    c->timing = enable ? 1 : 0;
+    return FX_OK;
+}
+
+extern "C" int fx_reset_timing(fx_context* c)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    for (double& v : c->ms) v = 0.0;
+    c->nl[0] = c->nl[1] = 0;
+    return FX_OK;
+}
+
+extern "C" int fx_get_timing(fx_context* c, double* ms4, int* nlaunch2)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    if (ms4)
+        for (int i = 0; i < 4; i++) ms4[i] = c->ms[i];
+    if (nlaunch2) {
+        nlaunch2[0] = c->nl[0];
+        nlaunch2[1] = c->nl[1];
+    }
+    return FX_OK;
+}
+
+extern "C" int fx_get_geometry(const fx_context* c, int* o)
+{
+    if (!c || !o) return fail(FX_EINVAL, "null argument");
+    const FxPlan& p = c->p;
+    const int v[11] = {p.F, p.stride, p.sub, p.nstep, p.nfstep, p.S, p.Spad, p.nblk, p.nbp, c->G, c->granule};
+    memcpy(o, v, sizeof(v));
+    return FX_OK;
+}

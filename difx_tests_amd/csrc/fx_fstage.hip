@@ -1,0 +1,666 @@
+// fx_fstage.hip -- F-stage of the MI355X FX engine (gfx950).
+//
+//   fx_delay_kernel   FP64 delay model per (FFT, antenna): kernel.c:373-386
+//                     (getStationDelay) and kernel.c:736-756 (integer/
+//                     fractional split, edge clamps, validity).
+//   fx_fstage_kernel  fused 2-bit unpack (LDS byte window + 256-entry LDS level
+//                     table), fringe rotation (kernel.c:428-497), forward C2C FFT
+//                     (kernel.c:499-510) as LDS Stockham passes with in-register
+//                     radix-8/16/32 butterflies, fractional-sample correction of
+//                     channels < nchan (kernel.c:512-555) and the conversion to
+//                     the X-engine's corner-turned input (FP32 / FP16 / FP8).
+//
+// One workgroup handles G consecutive FFTs of one antenna, both polarisations,
+// and keeps the G spectra of every channel it owns in registers, so each
+// (channel, signal) run of G FFTs leaves in one 16- or 32-byte store.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include "fx_common.h"
+#include "fx_kernels.h"
+
+namespace fx {
+
+static constexpr double kPi = 3.14159265358979323846;
+
+// --------------------------------------------------------------------------
+// FP64 delay table
+
+__global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
+                                const double* __restrict__ fileoff, const int* __restrict__ iteration,
+                                int numffts, int nffts_pad, int maxoffset, FxDelay* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nffts_pad * p.nant) return;
+    const int k = t / p.nant;
+    const int j = t - k * p.nant;
+    FxDelay e;
+    if (k >= numffts) {
+        e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
+        out[t] = e;
+        return;
+    }
+    const int fftindex = iteration[k];
+    const double c0 = delays[3 * j], c1 = delays[3 * j + 1], c2 = delays[3 * j + 2];
+    // kernel.c:378-385
+    const double d0 = c0 * fftindex * fftindex + c1 * fftindex + c2;
+    const double d1 = c0 * (fftindex + 0.5) * (fftindex + 0.5) + c1 * (fftindex + 0.5) + c2;
+    const double d2 = c0 * (fftindex + 1.0) * (fftindex + 1.0) + c1 * (fftindex + 1.0) + c2;
+    const double a = d2 - d0;
+    const double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+    const double mean = a * 0.5 + b;
+    // kernel.c:736-756
+    const double x = (mean - fileoff[j]) / p.Ts;
+    const int n = (int)(x + 0.5);
+    double fd = -(x - n) * p.Ts;
+    int o = fftindex * p.F - n;
+    if (o == -1) {
+        ++o;
+        fd += p.Ts;
+    }
+    if (o == maxoffset + 1) {
+        --o;
+        fd -= p.Ts;
+    }
+    e.a = a;
+    e.b = b;
+    e.fd = fd;
+    e.offset = o;
+    e.valid = !(o < 0 || o > maxoffset);
+    out[t] = e;
+}
+
+hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
+                        int numffts, int nffts_pad, int maxoffset, FxDelay* out, hipStream_t s)
+{
+    const int n = nffts_pad * p.nant;
+    if (n <= 0) return hipSuccess;
+    fx_delay_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, delays, fileoff, iteration, numffts, nffts_pad,
+                                                    maxoffset, out);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// complex helpers
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// a*b with every product and sum rounded separately (the reference's IPP
+// complex multiply); used where the oracle must agree bit for bit.
+__device__ __forceinline__ float2 cmul_exact(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// cos(2 pi q / 32) from the first quadrant by symmetry.
+__host__ __device__ constexpr float kc32(int q)
+{
+    constexpr float c[9] = {1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f,
+                            7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f,
+                            0.0f};
+    return q <= 8 ? c[q] : q <= 16 ? -c[16 - q] : q <= 24 ? -c[q - 16] : c[32 - q];
+}
+__host__ __device__ constexpr float ks32(int q) { return kc32((q + 24) & 31); }  // sin x = cos(x - pi/2)
+
+// x * exp(-2 pi i q / 32); q is a constant once the caller is unrolled.
+__device__ __forceinline__ float2 twc(float2 x, int q)
+{
+    if (q == 0) return x;
+    if (q == 8) return make_float2(x.y, -x.x);
+    if (q == 16) return make_float2(-x.x, -x.y);
+    if (q == 24) return make_float2(-x.y, x.x);
+    if (q == 4) {
+        const float r = 7.071067812e-01f;
+        return make_float2((x.x + x.y) * r, (x.y - x.x) * r);
+    }
+    if (q == 12) {
+        const float r = 7.071067812e-01f;
+        return make_float2((x.y - x.x) * r, -(x.x + x.y) * r);
+    }
+    const float c = kc32(q), s = ks32(q);
+    return make_float2(x.x * c + x.y * s, x.y * c - x.x * s);
+}
+
+// sin and cos of an FP32 argument, evaluated in FP64 and rounded to FP32 (the
+// oracle rounds the C library's double sin/cos the same way).  The rotator
+// arguments of a valid antenna are bounded (|x| < 8 rad): the reduction by
+// pi/2 is Cody-Waite with a three-part constant (exact for |x| < 2^20), the
+// kernels are the fdlibm minimax polynomials on [-pi/4, pi/4].
+__device__ __forceinline__ float2 sincos_rounded(float xf)
+{
+    const double x = (double)xf;
+    const double q = rint(x * 6.36619772367581382433e-01);       // 2/pi
+    double r = fma(-q, 1.57079632673412561417e+00, x);          // pi/2 high
+    r = fma(-q, 6.07710050650619224932e-11, r);                 // pi/2 mid
+    r = fma(-q, 2.02226624879595063154e-21, r);                 // pi/2 low
+    const double z = r * r;
+    const double sp = -1.66666666666666324348e-01 + z * (8.33333333332248946124e-03 +
+                      z * (-1.98412698298579493134e-04 + z * (2.75573137070700676789e-06 +
+                      z * (-2.50507602534068634195e-08 + z * 1.58969099521155010221e-10))));
+    const double cp = 4.16666666666666019037e-02 + z * (-1.38888888888741095749e-03 +
+                      z * (2.48015872894767294178e-05 + z * (-2.75573143513906633035e-07 +
+                      z * (2.08757232129817482790e-09 + z * -1.13596475577881948265e-11))));
+    const double sn = r + r * z * sp;
+    const double cs = 1.0 - 0.5 * z + z * z * cp;
+    const int quad = ((int)q) & 3;
+    double so, co;
+    switch (quad) {
+    case 0: so = sn; co = cs; break;
+    case 1: so = cs; co = -sn; break;
+    case 2: so = -sn; co = -cs; break;
+    default: so = -cs; co = sn; break;
+    }
+    return make_float2((float)co, (float)so);
+}
+
+__host__ __device__ constexpr int cbrev(int i, int bits)
+{
+    int r = 0;
+    for (int b = 0; b < bits; b++) r |= ((i >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+__host__ __device__ constexpr int clog2(int n) { return n <= 1 ? 0 : 1 + clog2(n >> 1); }
+
+// In-register forward DFT of size R <= 32, natural order in and out (radix-2
+// decimation in time, fully unrolled).  With HALF only outputs 0..R/2-1 are
+// formed: the last butterfly layer skips its difference half.
+template <int R, bool HALF = false>
+__device__ __forceinline__ void dft(float2 (&x)[R])
+{
+    constexpr int BITS = clog2(R);
+    float2 y[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) y[cbrev(i, BITS)] = x[i];
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int j = 0; j < len / 2; j++) {
+                const float2 u = y[i + j];
+                const float2 v = twc(y[i + j + len / 2], j * (32 / len));
+                y[i + j] = cadd(u, v);
+                if (!(HALF && len == R)) y[i + j + len / 2] = csub(u, v);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) x[i] = y[i];
+}
+
+// x[r] *= W^(r*e) for r = 1..R-1, W = exp(-2 pi i / F) from the table tw.
+// Each twiddle is one product of two loaded factors, W^(4ae) * W^(be) for
+// r = 4a + b, so a radix-R butterfly loads R/4 + 3 table entries.
+template <int R>
+__device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __restrict__ tw, int e)
+{
+    float2 lo[4], hi[R / 4];
+    lo[0] = make_float2(1.0f, 0.0f);
+#pragma unroll
+    for (int b = 1; b < 4 && b < R; b++) lo[b] = tw[b * e];
+    hi[0] = make_float2(1.0f, 0.0f);
+#pragma unroll
+    for (int a = 1; a < R / 4; a++) hi[a] = tw[4 * a * e];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+        const int a = r >> 2, b = r & 3;
+        float2 w;
+        if (a == 0) w = lo[b];
+        else if (b == 0) w = hi[a];
+        else w = cmul(hi[a], lo[b]);
+        x[r] = cmul(x[r], w);
+    }
+}
+
+// --------------------------------------------------------------------------
+// Compile-time geometry for F = 2^LG.  Passes R0 (fused unpack), R1, R2 (last,
+// lower half only); threads NT = F / R0 so pass 0 gives every thread one
+// butterfly of both polarisations.
+
+template <int LG> struct RPlan;
+template <> struct RPlan<9>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8; };
+template <> struct RPlan<10> { static constexpr int R0 = 16, R1 = 8,  R2 = 8; };
+template <> struct RPlan<11> { static constexpr int R0 = 16, R1 = 16, R2 = 8; };
+template <> struct RPlan<12> { static constexpr int R0 = 16, R1 = 16, R2 = 16; };
+template <> struct RPlan<13> { static constexpr int R0 = 16, R1 = 16, R2 = 32; };
+
+template <int LG> struct Geo {
+    static constexpr int F = 1 << LG;
+    static constexpr int NCH = F / 2;
+    static constexpr int STRIDE = 1 << ((LG - 1) / 2);     // 2^floor(log2(nchan)/2)
+    static constexpr int SUB = 2 * STRIDE;
+    static constexpr int NSTEP = F / SUB;
+    static constexpr int NFSTEP = NCH / STRIDE;
+    static constexpr int NTAB = SUB + NSTEP + STRIDE + NFSTEP;
+    static constexpr int R0 = RPlan<LG>::R0, R1 = RPlan<LG>::R1, R2 = RPlan<LG>::R2;
+    static constexpr int NT = F / R0;
+    static_assert(R0 * R1 * R2 == F, "plan must factor F");
+    static_assert(NT % SUB == 0, "pass-0 rotator reuse needs F/R0 to be a multiple of sub");
+};
+
+__device__ __forceinline__ int pad(int n) { return n + (n >> 4); }  // LDS bank skew
+
+// Per-thread storage of the G spectra of the channels a thread owns.
+template <int PREC, int G, int NOUT> struct OutRegs;
+// The G FFTs of a group pass through a register shift chain: after each FFT
+// the chain moves down one slot and the new spectrum enters slot G-1, so the
+// FFT loop can stay rolled while every register index is static.
+template <int G, int NOUT> struct OutRegs<FX_PREC_F32, G, NOUT> {
+    float2 v[G][NOUT];
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int g = 0; g + 1 < G; g++)
+#pragma unroll
+            for (int u = 0; u < NOUT; u++) v[g][u] = v[g + 1][u];
+    }
+    __device__ __forceinline__ void zero_last() {
+#pragma unroll
+        for (int u = 0; u < NOUT; u++) v[G - 1][u] = make_float2(0.0f, 0.0f);
+    }
+    __device__ __forceinline__ void put_last(int u, float2 x, float) { v[G - 1][u] = x; }
+};
+template <int G, int NOUT> struct OutRegs<FX_PREC_F16, G, NOUT> {
+    uint32_t v[G][NOUT];
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int g = 0; g + 1 < G; g++)
+#pragma unroll
+            for (int u = 0; u < NOUT; u++) v[g][u] = v[g + 1][u];
+    }
+    __device__ __forceinline__ void zero_last() {
+#pragma unroll
+        for (int u = 0; u < NOUT; u++) v[G - 1][u] = 0u;
+    }
+    __device__ __forceinline__ void put_last(int u, float2 x, float) {
+        const __half2 h = __floats2half2_rn(x.x, x.y);
+        v[G - 1][u] = *reinterpret_cast<const uint32_t*>(&h);
+    }
+};
+// FP8: a word holds two FFTs (re,im e4m3 each) as 16-bit halves, slot order
+// (low, high) = (earlier, later); the chain shifts by half-words.
+template <int G, int NOUT> struct OutRegs<FX_PREC_F8, G, NOUT> {
+    uint32_t v[G / 2][NOUT];
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int u = 0; u < NOUT; u++) {
+#pragma unroll
+            for (int w = 0; w + 1 < G / 2; w++) v[w][u] = (v[w][u] >> 16) | (v[w + 1][u] << 16);
+            v[G / 2 - 1][u] >>= 16;
+        }
+    }
+    __device__ __forceinline__ void zero_last() {}
+    __device__ __forceinline__ void put_last(int u, float2 x, float si) {
+        const float a = fminf(fmaxf(x.x * si, -448.0f), 448.0f);
+        const float b = fminf(fmaxf(x.y * si, -448.0f), 448.0f);
+        v[G / 2 - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, (int)v[G / 2 - 1][u], true);
+    }
+};
+
+template <int PREC> struct GroupSize;
+template <> struct GroupSize<FX_PREC_F32> { static constexpr int G = 2; };
+template <> struct GroupSize<FX_PREC_F16> { static constexpr int G = 4; };
+template <> struct GroupSize<FX_PREC_F8>  { static constexpr int G = 8; };
+
+int fx_fstage_group_size(int prec)
+{
+    return prec == FX_PREC_F8 ? GroupSize<FX_PREC_F8>::G
+         : prec == FX_PREC_F16 ? GroupSize<FX_PREC_F16>::G : GroupSize<FX_PREC_F32>::G;
+}
+
+// --------------------------------------------------------------------------
+// The fused F-stage kernel.
+
+template <int LG, int PREC, bool TAP>
+__global__ __launch_bounds__(Geo<LG>::NT) void fx_fstage_kernel(FStageArgs args)
+{
+    using Gm = Geo<LG>;
+    constexpr int F = Gm::F, NCH = Gm::NCH, NT = Gm::NT;
+    constexpr int STRIDE = Gm::STRIDE, SUB = Gm::SUB, NSTEP = Gm::NSTEP, NFSTEP = Gm::NFSTEP;
+    constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2;
+    constexpr int G = GroupSize<PREC>::G;
+    constexpr int T0 = F / R0, T1 = F / R1, T2 = F / R2;   // butterflies per pol per pass
+    constexpr int P1 = R0, P2 = R0 * R1;                    // sub-transform lengths
+    constexpr int PER1 = 2 * T1 / NT, PER2 = 2 * T2 / NT;   // (pol, butterfly) units per thread
+    static_assert(PER1 * NT == 2 * T1 && PER2 * NT == 2 * T2, "plan/thread mismatch");
+    static_assert(T0 == NT, "pass 0 maps one butterfly to each thread");
+    constexpr int NOUT = PER2 * (R2 / 2);
+    constexpr int FP = F + F / 16;
+
+    __shared__ __align__(16) float2 buf[2 * FP];
+    __shared__ __align__(16) float4 lut[256];
+    __shared__ __align__(16) uint32_t bytes[F / 8 + 4];
+    __shared__ float2 rsub[SUB], rstep[NSTEP], frsub[STRIDE], frstep[NFSTEP];
+    __shared__ float2 tw1[P1 * R1];                 // pass-1 twiddles exp(-2 pi i m / (P1 R1))
+
+    const FxPlan& p = args.p;
+    const int tid = threadIdx.x;
+
+    // XCD-aware order: hardware places block b on XCD b % 8; give each XCD a
+    // contiguous range of (group, antenna) work so the G-FFT runs written by
+    // neighbouring antennas meet in the same L2.
+    const int nblocks = gridDim.x;
+    const int q8 = nblocks >> 3, r8 = nblocks & 7;
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
+    const int grp = lid / p.nant;
+    const int ant = lid - grp * p.nant;
+    const int fgo = args.fg0 + grp;
+
+    for (int e = tid; e < P1 * R1; e += NT) tw1[e] = args.tw[e * (F / (P1 * R1))];
+    for (int e = tid; e < 256; e += NT) {
+        const float hi = 3.3359f;
+        const float lv[4] = {-hi, -1.0f, 1.0f, hi};
+        lut[e] = make_float4(lv[e & 3], lv[(e >> 2) & 3], lv[(e >> 4) & 3], lv[(e >> 6) & 3]);
+    }
+
+    const uint8_t* src = args.data + (long long)ant * args.data_stride;
+    OutRegs<PREC, G, NOUT> outr;
+
+#pragma unroll 1
+    for (int gi = 0; gi < G; gi++) {
+        outr.shift();
+        const int fft = grp * G + gi;             // FFT of this launch
+        const FxDelay d = args.delays[(long long)fft * p.nant + ant];
+        __syncthreads();                          // previous FFT done with LDS
+        if (!d.valid) {                           // uniform across the block
+            outr.zero_last();
+            if (TAP) {
+#pragma unroll
+                for (int m = 0; m < PER2; m++) {
+                    const int u = tid + m * NT;
+                    const int pol = u / T2, i = u % T2;
+#pragma unroll
+                    for (int r = 0; r < R2 / 2; r++)
+                        args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + i + r * P2] =
+                            make_float2(0.0f, 0.0f);
+                }
+            }
+            continue;
+        }
+        // ---- rotator tables: FP64 phase arguments, FP32 rotators
+        {
+#pragma clang fp contract(off)
+            const int integerdelay = (int)d.b;
+            const double bfrac = d.b - integerdelay;
+            const double lofr = p.lofrac * (double)integerdelay;
+            const float fd32 = (float)d.fd;
+            for (int e = tid; e < Gm::NTAB; e += NT) {
+                float arg;
+                bool unit = false;
+                float2* dst;
+                if (e < SUB) {
+                    double subxval = ((double)e / (double)F) * d.a;
+                    subxval += bfrac;
+                    double ph = subxval * p.lo;
+                    if (p.fracLO) ph += lofr;
+                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
+                    dst = rsub + e;
+                } else if (e < SUB + NSTEP) {
+                    const int k = e - SUB;
+                    const double stepxval = (((double)k * (double)STRIDE) / (double)F) * d.a;
+                    const double ph = stepxval * p.lo;
+                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
+                    unit = (k == 0);
+                    dst = rstep + k;
+                } else if (e < SUB + NSTEP + STRIDE) {
+                    const int s = e - SUB - NSTEP;
+                    arg = args.subchf[s] * fd32;
+                    dst = frsub + s;
+                } else {
+                    const int k = e - SUB - NSTEP - STRIDE;
+                    arg = args.stepchf[k] * fd32;
+                    unit = (k == 0);
+                    dst = frstep + k;
+                }
+                *dst = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
+            }
+        }
+        // ---- packed byte window of this FFT
+        const int o = d.offset;
+        const int bbase = (o >> 1) & ~3;
+        const int nd = ((((o + F - 1) >> 1) - bbase) >> 2) + 1;
+        const uint32_t* src32 = reinterpret_cast<const uint32_t*>(src + bbase);
+        for (int w = tid; w < nd; w += NT) bytes[w] = src32[w];
+        __syncthreads();
+
+        // ---- pass 0: unpack + fringe rotate + radix-R0 butterfly, both pols
+        {
+            const uint8_t* bl = reinterpret_cast<const uint8_t*>(bytes);
+            const int i = tid;
+            const float2 rs = rsub[i & (SUB - 1)];
+            float2 x0[R0], x1[R0];
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                const int n = i + r * T0;
+                const float2 rot = cmul_exact(rs, rstep[n / SUB]);
+                const int q = o + n;
+                const float4 lv = lut[bl[(q >> 1) - bbase]];
+                const float l0 = (q & 1) ? lv.z : lv.x;
+                const float l1 = (q & 1) ? lv.w : lv.y;
+                x0[r] = make_float2(l0 * rot.x, l0 * rot.y);
+                x1[r] = make_float2(l1 * rot.x, l1 * rot.y);
+            }
+            dft<R0>(x0);
+            dft<R0>(x1);
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                buf[pad(i * R0 + r)] = x0[r];
+                buf[FP + pad(i * R0 + r)] = x1[r];
+            }
+        }
+        __syncthreads();
+
+        // ---- pass 1: radix R1 on sub-transforms of length P1
+        {
+            float2 x[PER1][R1];
+#pragma unroll
+            for (int m = 0; m < PER1; m++) {
+                const int u = tid + m * NT;
+                const int pol = u / T1, i = u % T1;
+                const int k = i & (P1 - 1);
+#pragma unroll
+                for (int r = 0; r < R1; r++) x[m][r] = buf[pol * FP + pad(i + r * T1)];
+#pragma unroll
+                for (int r = 1; r < R1; r++) x[m][r] = cmul(x[m][r], tw1[r * k]);
+                dft<R1>(x[m]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < PER1; m++) {
+                const int u = tid + m * NT;
+                const int pol = u / T1, i = u % T1;
+                const int k = i & (P1 - 1);
+                const int j = (i - k) * R1 + k;
+#pragma unroll
+                for (int r = 0; r < R1; r++) buf[pol * FP + pad(j + r * P1)] = x[m][r];
+            }
+        }
+        __syncthreads();
+
+        // ---- pass 2 (last): radix R2, lower half only, frac-sample rotation
+#pragma unroll
+        for (int m = 0; m < PER2; m++) {
+            const int u = tid + m * NT;
+            const int pol = u / T2, i = u % T2;
+            float2 x[R2];
+#pragma unroll
+            for (int r = 0; r < R2; r++) x[r] = buf[pol * FP + pad(i + r * T2)];
+            twiddle_pow<R2>(x, args.tw, i);
+            dft<R2, true>(x);
+#pragma unroll
+            for (int r = 0; r < R2 / 2; r++) {
+                const int c = i + r * P2;
+                const float2 fr = cmul_exact(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
+                const float2 y = cmul_exact(fr, x[r]);
+                float si = 1.0f;
+                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + 2 * ant + pol];
+                outr.put_last(m * (R2 / 2) + r, y, si);
+                if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
+            }
+        }
+    }
+
+    // ---- stores: corner-turned [chan][group][Spad][G]
+#pragma unroll
+    for (int m = 0; m < PER2; m++) {
+        const int u = tid + m * NT;
+        const int pol = u / T2, i = u % T2;
+        const int sig = 2 * ant + pol;
+#pragma unroll
+        for (int r = 0; r < R2 / 2; r++) {
+            const int c = i + r * P2;
+            const int ou = m * (R2 / 2) + r;
+            const long long row = ((long long)c * args.nfg + fgo) * p.Spad + sig;
+            if constexpr (PREC == FX_PREC_F16) {
+                uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
+#pragma unroll
+                for (int q = 0; q < G / 4; q++)
+                    dst[q] = make_uint4(outr.v[4 * q][ou], outr.v[4 * q + 1][ou], outr.v[4 * q + 2][ou],
+                                        outr.v[4 * q + 3][ou]);
+            } else if constexpr (PREC == FX_PREC_F32) {
+                static_assert(G == 2, "FP32 rows are (re0, re1, im0, im1)");
+                float4* dst = reinterpret_cast<float4*>(args.out) + row;
+                *dst = make_float4(outr.v[0][ou].x, outr.v[1][ou].x, outr.v[0][ou].y, outr.v[1][ou].y);
+            } else {
+                uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 8);
+#pragma unroll
+                for (int q = 0; q < G / 8; q++)
+                    dst[q] = make_uint4(outr.v[4 * q][ou], outr.v[4 * q + 1][ou], outr.v[4 * q + 2][ou],
+                                        outr.v[4 * q + 3][ou]);
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Generic forward C2C FFT (ippsFFTFwd_CToC_32fc semantics: unnormalised, all
+// F bins), one workgroup per transform; same passes as the F-stage.
+
+template <int LG>
+__global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* __restrict__ in,
+                                                                 float2* __restrict__ out,
+                                                                 const float2* __restrict__ tw)
+{
+    using Gm = Geo<LG>;
+    constexpr int F = Gm::F, NT = Gm::NT, R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2;
+    constexpr int T0 = F / R0, T1 = F / R1, T2 = F / R2, P1 = R0, P2 = R0 * R1;
+    constexpr int PER1 = (T1 + NT - 1) / NT, PER2 = (T2 + NT - 1) / NT;
+    __shared__ __align__(16) float2 buf[F + F / 16];
+    const float2* x_in = in + (long long)blockIdx.x * F;
+    float2* x_out = out + (long long)blockIdx.x * F;
+    const int tid = threadIdx.x;
+    {
+        float2 x[R0];
+#pragma unroll
+        for (int r = 0; r < R0; r++) x[r] = x_in[tid + r * T0];
+        dft<R0>(x);
+#pragma unroll
+        for (int r = 0; r < R0; r++) buf[pad(tid * R0 + r)] = x[r];
+    }
+    __syncthreads();
+    {
+        float2 x[PER1][R1];
+#pragma unroll
+        for (int m = 0; m < PER1; m++) {
+            const int i = tid + m * NT, k = i & (P1 - 1);
+            if (i >= T1) continue;
+#pragma unroll
+            for (int r = 0; r < R1; r++) x[m][r] = buf[pad(i + r * T1)];
+#pragma unroll
+            for (int r = 1; r < R1; r++) x[m][r] = cmul(x[m][r], tw[(r * k) * (F / (P1 * R1))]);
+            dft<R1>(x[m]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < PER1; m++) {
+            const int i = tid + m * NT, k = i & (P1 - 1), j = (i - k) * R1 + k;
+            if (i >= T1) continue;
+#pragma unroll
+            for (int r = 0; r < R1; r++) buf[pad(j + r * P1)] = x[m][r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER2; m++) {
+        const int i = tid + m * NT;
+        if (i >= T2) continue;
+        float2 x[R2];
+#pragma unroll
+        for (int r = 0; r < R2; r++) x[r] = buf[pad(i + r * T2)];
+#pragma unroll
+        for (int r = 1; r < R2; r++) x[r] = cmul(x[r], tw[r * i]);
+        dft<R2>(x);
+#pragma unroll
+        for (int r = 0; r < R2; r++) x_out[i + r * P2] = x[r];
+    }
+}
+
+template <int LG>
+static hipError_t launch_fft_lg(const float2* in, float2* out, const float2* tw, int batch, hipStream_t s)
+{
+    fx_fft_c2c_kernel<LG><<<batch, Geo<LG>::NT, 0, s>>>(in, out, tw);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int lg, int batch, hipStream_t s)
+{
+    if (batch <= 0) return hipSuccess;
+    switch (lg) {
+    case 9:  return launch_fft_lg<9>(in, out, tw, batch, s);
+    case 10: return launch_fft_lg<10>(in, out, tw, batch, s);
+    case 11: return launch_fft_lg<11>(in, out, tw, batch, s);
+    case 12: return launch_fft_lg<12>(in, out, tw, batch, s);
+    case 13: return launch_fft_lg<13>(in, out, tw, batch, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool fstage_supported(int lg) { return lg >= 9 && lg <= 13; }
+
+template <int LG>
+static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
+{
+    const int nblocks = a.ngroups * a.p.nant;
+    if (nblocks <= 0) return hipSuccess;
+    const dim3 grid(nblocks), block(Geo<LG>::NT);
+    const bool tap = a.tap != nullptr;
+    switch (prec) {
+    case FX_PREC_F32:
+        if (tap) fx_fstage_kernel<LG, FX_PREC_F32, true><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, FX_PREC_F32, false><<<grid, block, 0, s>>>(a);
+        break;
+    case FX_PREC_F16:
+        if (tap) fx_fstage_kernel<LG, FX_PREC_F16, true><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, FX_PREC_F16, false><<<grid, block, 0, s>>>(a);
+        break;
+    case FX_PREC_F8:
+        if (tap) fx_fstage_kernel<LG, FX_PREC_F8, true><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, FX_PREC_F8, false><<<grid, block, 0, s>>>(a);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
+{
+    switch (a.p.lg) {
+    case 9:  return launch_lg<9>(a, prec, s);
+    case 10: return launch_lg<10>(a, prec, s);
+    case 11: return launch_lg<11>(a, prec, s);
+    case 12: return launch_lg<12>(a, prec, s);
+    case 13: return launch_lg<13>(a, prec, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace fx

@@ -1,0 +1,52 @@
+// fx_kernels.h -- launch interface between the context code and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "fx_common.h"
+
+namespace fx {
+
+struct FStageArgs {
+    FxPlan p;
+    const uint8_t* data;          // packed input, antenna a at data + a*data_stride
+    long long data_stride;
+    const FxDelay* delays;        // [nffts][nant] for this call
+    const float2* tw;             // [F] exp(-2 pi i m / F)
+    const float* subchf;          // [stride] f32 channel frequencies (kernel.c:164)
+    const float* stepchf;         // [nfstep] (kernel.c:165)
+    void* out;                    // X input, [nchan][nfg][Spad][G]
+    int nfg;                      // groups held by the X input buffer
+    int fg0;                      // first group written by this launch
+    int ngroups;                  // groups in this launch
+    const float* scale_inv;       // FP8: [nchan][Spad] inverse scales
+    float2* tap;                  // optional FP32 spectra [nffts][nant][2][nchan]
+};
+
+struct XArgs {
+    FxPlan p;
+    const void* xin;              // [nchan][nfg][Spad][G]
+    int nfg;                      // groups held by the buffer (row stride)
+    int fg0, ngroups;             // group range reduced by this launch
+    float2* vacc;                 // [nchan][nbp][32*32]
+    const int4* tasks;            // (I, J0, nblocks, bp0) per wave task
+    int ntasks;
+    int ngroups_wg;               // wave tasks per workgroup
+    int accumulate;               // 0: overwrite, 1: add
+    const float* scale;           // FP8: [nchan][Spad] scales
+};
+
+// F-stage: LG = log2(F); prec = FX_PREC_*; G is fixed per precision.
+int fx_fstage_group_size(int prec);
+hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
+                        int numffts, int nffts_pad, int maxoffset, FxDelay* out, hipStream_t s);
+hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
+bool fstage_supported(int lg);
+// Batched forward C2C FFT of length 2^lg (natural order in/out, unnormalised).
+hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int lg, int batch, hipStream_t s);
+
+hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s);
+hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s);
+hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, hipStream_t s);
+hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
+
+}  // namespace fx

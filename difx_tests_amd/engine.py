@@ -1,0 +1,204 @@
+"""ctypes bindings of the native FX engine (include/difxfx.h).
+
+The engine is libdifxfx.so built in-tree by difx_tests_amd.build; nothing
+here computes visibilities on the CPU.  Loading fails loudly when the library
+is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIBPATH = os.path.join(LIBDIR, "libdifxfx.so")
+
+PREC_F32, PREC_F16, PREC_F8 = 0, 1, 2
+PREC_NAMES = {"f32": PREC_F32, "f16": PREC_F16, "f8": PREC_F8}
+
+# Every symbol include/difxfx.h declares.
+NATIVE_SYMBOLS = [
+    "fx_create", "fx_destroy", "fx_last_error", "fx_fft_granule", "fx_set_model", "fx_upload_data",
+    "fx_upload_data_strided", "fx_device_data", "fx_fstage", "fx_xstage", "fx_process", "fx_reset",
+    "fx_read_vis", "fx_read_delay_table", "fx_enable_tap", "fx_read_tap", "fx_set_fp8_scales", "fx_sync",
+    "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry",
+]
+# Every symbol include/kernel.h declares (kernel.h:61-134 of the reference).
+KERNELH_SYMBOLS = [
+    "initLUT2bitReal", "init_FxKernel", "unpack", "getStationDelay", "unpackReal2bit", "fringerotate", "dofft",
+    "fracSampleCorrect", "conjChannels", "saveVisibilities", "startTiming", "saveLog", "endTiming",
+    "processAntennas", "processNormalize", "processBaseline", "processAntennasAndBaseline",
+    "processAntennasAndBaselinePara", "resetBeforeProcess", "manageAntennasAndBaseline", "processBaselinePara",
+    "merge", "unpackImpl", "fringeRotateImpl", "doFFTImpl", "fracSampleCorrectImpl", "conjChannelsImpl",
+    "stationDelayAndOffset",
+]
+
+DELAY_DTYPE = np.dtype([("a", "<f8"), ("b", "<f8"), ("fd", "<f8"), ("offset", "<i4"), ("valid", "<i4")])
+
+
+class FxError(RuntimeError):
+    pass
+
+
+class _Params(C.Structure):
+    _fields_ = [("nant", C.c_int), ("nchan", C.c_int), ("lo", C.c_double), ("bw", C.c_double),
+                ("prec", C.c_int), ("max_ffts", C.c_int), ("max_bytes_per_ant", C.c_longlong)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIBPATH) -> C.CDLL:
+    """Load libdifxfx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FxError(f"{path} is missing: run `python -m difx_tests_amd.build` first")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    I = C.c_int
+    lib.fx_create.argtypes = [C.POINTER(_Params), C.POINTER(P)]
+    lib.fx_destroy.argtypes = [P]
+    lib.fx_destroy.restype = None
+    lib.fx_last_error.restype = C.c_char_p
+    lib.fx_fft_granule.argtypes = [P]
+    lib.fx_set_model.argtypes = [P, P, P]
+    lib.fx_upload_data.argtypes = [P, P, C.c_longlong]
+    lib.fx_upload_data_strided.argtypes = [P, P, C.c_longlong, C.c_longlong]
+    lib.fx_device_data.argtypes = [P, C.POINTER(P), C.POINTER(C.c_longlong)]
+    lib.fx_fstage.argtypes = [P, I, P, I, I, C.POINTER(I)]
+    lib.fx_xstage.argtypes = [P, I, I]
+    lib.fx_process.argtypes = [P, I, P, I]
+    lib.fx_reset.argtypes = [P]
+    lib.fx_read_vis.argtypes = [P, P, P, P, P]
+    lib.fx_read_delay_table.argtypes = [P, I, P]
+    lib.fx_enable_tap.argtypes = [P, I]
+    lib.fx_read_tap.argtypes = [P, I, P]
+    lib.fx_set_fp8_scales.argtypes = [P, P]
+    lib.fx_sync.argtypes = [P]
+    lib.fx_set_timing.argtypes = [P, I]
+    lib.fx_reset_timing.argtypes = [P]
+    lib.fx_get_timing.argtypes = [P, P, P]
+    lib.fx_get_geometry.argtypes = [P, P]
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+class FxEngine:
+    """One subband on the GPU: delay model, packed input, F-stage, X-engine."""
+
+    def __init__(self, nant: int, nchan: int, lo: float, bw: float, prec: int | str = PREC_F32,
+                 max_ffts: int = 64, max_bytes_per_ant: int = 0):
+        self.lib = load_library()
+        if isinstance(prec, str):
+            prec = PREC_NAMES[prec]
+        self.nant, self.nchan, self.lo, self.bw, self.prec = nant, nchan, float(lo), float(bw), prec
+        prm = _Params(nant, nchan, float(lo), float(bw), prec, max_ffts, max_bytes_per_ant)
+        h = C.c_void_p()
+        self._check(self.lib.fx_create(C.byref(prm), C.byref(h)))
+        self.h = h
+        geo = np.zeros(11, np.int32)
+        self._check(self.lib.fx_get_geometry(self.h, _ptr(geo)))
+        (self.F, self.stride, self.sub, self.nstep, self.nfstep, self.S, self.Spad, self.nblk, self.nbp,
+         self.G, self.granule) = [int(v) for v in geo]
+        self.nbl = nant * (nant - 1) // 2
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise FxError(f"libdifxfx error {rc}: {self.lib.fx_last_error().decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.fx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- inputs
+    def set_model(self, delays: np.ndarray, fileoffsets: np.ndarray) -> None:
+        d = np.ascontiguousarray(delays, dtype=np.float64).reshape(self.nant, 3)
+        o = np.ascontiguousarray(fileoffsets, dtype=np.float64).reshape(self.nant)
+        self._check(self.lib.fx_set_model(self.h, _ptr(d), _ptr(o)))
+
+    def upload(self, data: np.ndarray) -> None:
+        """data: uint8 [nant][nbytes] packed 2-bit samples."""
+        d = np.ascontiguousarray(data, dtype=np.uint8)
+        assert d.ndim == 2 and d.shape[0] == self.nant
+        self._check(self.lib.fx_upload_data_strided(self.h, _ptr(d), d.shape[1], d.shape[1]))
+
+    # ---- pipeline
+    def fstage(self, iteration, split: int = 1, fft0: int = 0) -> int:
+        it = np.ascontiguousarray(iteration, dtype=np.int32)
+        ns = C.c_int()
+        self._check(self.lib.fx_fstage(self.h, len(it), _ptr(it), split, fft0, C.byref(ns)))
+        return ns.value
+
+    def xstage(self, fft0: int, nslots: int) -> None:
+        self._check(self.lib.fx_xstage(self.h, fft0, nslots))
+
+    def process(self, iteration, split: int = 1) -> None:
+        it = np.ascontiguousarray(iteration, dtype=np.int32)
+        self._check(self.lib.fx_process(self.h, len(it), _ptr(it), split))
+
+    def reset(self) -> None:
+        self._check(self.lib.fx_reset(self.h))
+
+    def sync(self) -> None:
+        self._check(self.lib.fx_sync(self.h))
+
+    # ---- outputs
+    def read_vis(self):
+        """(cross [nbl,4,nchan] c64, counts [nbl], autos [nant,4,nchan] c64, autocounts [nant])."""
+        cross = np.zeros((max(self.nbl, 1), 4, self.nchan), np.complex64)
+        autos = np.zeros((self.nant, 4, self.nchan), np.complex64)
+        cc = np.zeros(max(self.nbl, 1), np.int32)
+        ac = np.zeros(self.nant, np.int32)
+        self._check(self.lib.fx_read_vis(self.h, _ptr(cross), _ptr(cc), _ptr(autos), _ptr(ac)))
+        return cross[: self.nbl], cc[: self.nbl], autos, ac
+
+    def read_delay_table(self, nffts: int) -> np.ndarray:
+        out = np.zeros(nffts * self.nant, DELAY_DTYPE)
+        self._check(self.lib.fx_read_delay_table(self.h, nffts, _ptr(out)))
+        return out.reshape(nffts, self.nant)
+
+    def enable_tap(self, on: bool = True) -> None:
+        self._check(self.lib.fx_enable_tap(self.h, 1 if on else 0))
+
+    def read_tap(self, nffts: int) -> np.ndarray:
+        out = np.zeros((nffts, self.nant, 2, self.nchan), np.complex64)
+        self._check(self.lib.fx_read_tap(self.h, nffts, _ptr(out)))
+        return out
+
+    def set_fp8_scales(self, scales: np.ndarray) -> None:
+        s = np.ascontiguousarray(scales, dtype=np.float32).reshape(self.nchan, 2 * self.nant)
+        self._check(self.lib.fx_set_fp8_scales(self.h, _ptr(s)))
+
+    # ---- timing
+    def set_timing(self, on: bool = True) -> None:
+        self._check(self.lib.fx_set_timing(self.h, 1 if on else 0))
+
+    def reset_timing(self) -> None:
+        self._check(self.lib.fx_reset_timing(self.h))
+
+    def timing(self):
+        """({delay, fstage, counts, xstage} ms summed, {fstage, xstage} launch counts)."""
+        ms = np.zeros(4, np.float64)
+        nl = np.zeros(2, np.int32)
+        self._check(self.lib.fx_get_timing(self.h, _ptr(ms), _ptr(nl)))
+        return dict(zip(("delay", "fstage", "counts", "xstage"), ms.tolist())), nl.tolist()

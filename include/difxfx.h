@@ -1,0 +1,123 @@
+/*
+ * difxfx.h -- native C API of the MI355X FX correlator engine.
+ *
+ * Plain C ABI: pointers, sizes and status codes, no library types in the
+ * signatures.  Every function returns 0 on success and a negative code on
+ * failure (fx_last_error() gives the message).  A context owns the device
+ * buffers of one subband and one HIP stream; calls on one context must not
+ * overlap (the reference path is single threaded as well).
+ *
+ * The kernel.h entry points of the reference (kernel.h:61-134) are layered
+ * on top of this API; see kernel.h in this directory.
+ *
+ * Pipeline of one call (reference: processAntennasAndBaselinePara,
+ * kernel.c:709-827):
+ *   fx_fstage   FP64 delay table -> fused unpack / fringe rotation / FFT /
+ *               fractional-delay kernel -> corner-turned spectra in HBM
+ *   fx_xstage   MFMA X-engine: accumulate the Hermitian visibility matrix
+ *               of every channel over a range of FFT slots
+ *   fx_read_vis reference-layout cross visibilities [b][2l+m][chan], the
+ *               valid-FFT counts per baseline, and the autocorrelations
+ */
+#ifndef DIFXFX_H
+#define DIFXFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FX_PREC_F32 0   /* spectra and X-engine in FP32 (plain path)            */
+#define FX_PREC_F16 1   /* spectra cast to FP16, MFMA with FP32 accumulation     */
+#define FX_PREC_F8  2   /* spectra scaled per (channel, signal) to FP8 e4m3      */
+
+typedef struct fx_context fx_context;
+
+typedef struct {
+    int nant;                    /* antennas (2 polarisations each)                */
+    int nchan;                   /* channels; FFT length F = 2*nchan, power of two */
+    double lo;                   /* LO frequency, Hz (< 2^31, kernel.c:39)         */
+    double bw;                   /* bandwidth, Hz; sample time 1/(2*bw)            */
+    int prec;                    /* FX_PREC_*                                      */
+    int max_ffts;                /* FFT slots held for the X-engine                */
+    long long max_bytes_per_ant; /* packed input capacity per antenna              */
+} fx_params;
+
+/* Status codes */
+#define FX_OK 0
+#define FX_EINVAL -1
+#define FX_EHIP -2
+#define FX_ENOMEM -3
+#define FX_EUNSUPPORTED -4
+
+int fx_create(const fx_params *prm, fx_context **out);
+void fx_destroy(fx_context *ctx);
+const char *fx_last_error(void);
+
+/* FFT slots are grouped: fx_fstage writes whole K steps of this many FFTs. */
+int fx_fft_granule(const fx_context *ctx);
+
+/* Delay model: delays[3*a + {0,1,2}] = c0, c1, c2 (seconds, t in FFT units;
+ * kernel.c:378), fileoffsets[a] in seconds (loadConfig.h:30-31). */
+int fx_set_model(fx_context *ctx, const double *delays, const double *fileoffsets);
+
+/* Packed 2-bit input, host -> device.  data[a] holds nbytes bytes. */
+int fx_upload_data(fx_context *ctx, const uint8_t *const *data, long long nbytes);
+/* Same from one host buffer with antenna a at data + a*stride. */
+int fx_upload_data_strided(fx_context *ctx, const uint8_t *data, long long stride, long long nbytes);
+/* Device address and per-antenna stride of the packed input buffer. */
+int fx_device_data(fx_context *ctx, void **dptr, long long *stride);
+
+/* F-stage for numffts FFTs with global indices iteration[0..numffts)
+ * (host array), maximum offset (numffts*split - 1)*F (kernel.c:730), writing
+ * FFT slots [fft0, fft0 + padded numffts); fft0 must be a multiple of the
+ * granule.  *nslots receives the number of slots written. */
+int fx_fstage(fx_context *ctx, int numffts, const int *iteration, int split, int fft0, int *nslots);
+/* X-engine over FFT slots [fft0, fft0 + nslots), accumulated into the
+ * context's visibilities. */
+int fx_xstage(fx_context *ctx, int fft0, int nslots);
+/* fx_fstage at slot 0 followed by fx_xstage over the slots it wrote. */
+int fx_process(fx_context *ctx, int numffts, const int *iteration, int split);
+
+/* Zero the accumulated visibilities and counts. */
+int fx_reset(fx_context *ctx);
+
+/* Copy out the accumulated (unnormalised) visibilities:
+ *   cross[((b*4 + p)*nchan + c)*2 + {re,im}]  b over j<k j-major, p = 2l+m
+ *   cross_counts[b]  FFTs where both antennas were valid
+ *   autos[((a*4 + p)*nchan + c)*2 + {re,im}], auto_counts[a]
+ * Any output pointer may be NULL. */
+int fx_read_vis(fx_context *ctx, float *cross, int *cross_counts, float *autos, int *auto_counts);
+
+/* Debug taps (host copies).  Delay table entries are 32 bytes:
+ * {double a, b, fd; int offset, valid} per (FFT, antenna) of the last
+ * fx_fstage call. */
+int fx_read_delay_table(fx_context *ctx, int nffts, void *out);
+/* When enabled, fx_fstage also writes FP32 spectra [fft][ant][pol][nchan]
+ * (complex) of its FFTs; fx_read_tap copies nffts of them. */
+int fx_enable_tap(fx_context *ctx, int enable);
+int fx_read_tap(fx_context *ctx, int nffts, float *out);
+
+/* FP8: per (channel, signal) scales, scales[c*2*nant + 2*a + pol]; the
+ * stored value is spectrum/scale.  Defaults to the analytic 2-bit power. */
+int fx_set_fp8_scales(fx_context *ctx, const float *scales);
+
+int fx_sync(fx_context *ctx);
+
+/* Kernel timing with HIP events on the context's stream: when enabled, every
+ * fx_fstage / fx_xstage records events and fx_get_timing returns the summed
+ * milliseconds since the last fx_reset_timing of {delay, fstage, counts,
+ * xstage} and the launch counts of {fstage, xstage}. */
+int fx_set_timing(fx_context *ctx, int enable);
+int fx_reset_timing(fx_context *ctx);
+int fx_get_timing(fx_context *ctx, double *ms4, int *nlaunch2);
+
+/* Geometry used by the kernels (for reports): fills
+ * {F, stride, sub, nstep, nfstep, S, Spad, nblk, nbp, G, granule}. */
+int fx_get_geometry(const fx_context *ctx, int *out11);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

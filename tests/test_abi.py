@@ -1,0 +1,68 @@
+"""CPU checks of the drop-in boundary: the shared library loads without a GPU
+and exports every function the public headers declare; the config grammar
+matches the reference's parser (main.c:14-129)."""
+import os
+import re
+
+import pytest
+
+from difx_tests_amd import config, engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "for", "return", "sizeof")))
+
+
+@pytest.mark.parametrize("header", ["kernel.h", "difxfx.h"])
+def test_library_exports_every_declared_symbol(header):
+    lib = engine.load_library()
+    declared = _declared_functions(header)
+    assert len(declared) >= (28 if header == "kernel.h" else 20)
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, f"{header}: not exported: {missing}"
+
+
+def test_symbol_lists_match_headers():
+    assert sorted(engine.KERNELH_SYMBOLS) == _declared_functions("kernel.h")
+    assert sorted(engine.NATIVE_SYMBOLS) == _declared_functions("difxfx.h")
+
+
+def test_reference_config_fixture_parses():
+    # tests/golden/test4.conf is the reference's 4-antenna config (test4.conf)
+    cfg = config.parse_config(os.path.join(GOLDEN, "test4.conf"), warn=None)
+    assert (cfg.nbit, cfg.iscomplex, cfg.nchan, cfg.numffts, cfg.npol, cfg.nant) == (2, 0, 256, 3840, 2, 4)
+    assert cfg.lo == 1650000000.0 and cfg.bandwidth == 32000000.0
+    assert cfg.antennas[0].name == "Aa" and cfg.antennas[0].file == "aa.bin"
+    assert cfg.antennas[3].delay == (3.9e-19, 9.1e-15, 129.9e-9) and cfg.antennas[3].fileoffset == 130e-9
+    # allocDataHost.c:21: nchan * 2 * (numffts + 1) * 2 / 8 * 2
+    assert cfg.subintbytes == 256 * 2 * 3841 * 2 // 8 * 2
+
+
+def test_config_grammar_rules():
+    txt = "nbit 2\nNchan 64\nlo 1e9\nbandwidth 8e6\nNUMFFTS 4\nFOO 3\nNANT 1\nA a.bin 0 0 1e-7 1e-7\n"
+    cfg = config.parse_config_text(txt, warn=None)
+    assert (cfg.nchan, cfg.numffts, cfg.nant, cfg.npol, cfg.iscomplex) == (64, 4, 1, 2, 0)
+    with pytest.raises(config.ConfigError):          # empty line exits (main.c:37-40)
+        config.parse_config_text("NBIT 2\n\nNCHAN 4\n", warn=None)
+    with pytest.raises(config.ConfigError):          # missing value
+        config.parse_config_text("NCHAN\n", warn=None)
+    with pytest.raises(config.ConfigError):          # short antenna line
+        config.parse_config_text("NANT 1\nA a.bin 0 0 1e-7\n", warn=None)
+    again = config.parse_config_text(config.format_config(cfg), warn=None)
+    assert again == cfg
+
+
+def test_engine_fails_loudly_without_library(tmp_path):
+    engine._lib_backup = engine._lib
+    try:
+        engine._lib = None
+        with pytest.raises(engine.FxError):
+            engine.load_library(str(tmp_path / "missing.so"))
+    finally:
+        engine._lib = engine._lib_backup

@@ -1,0 +1,196 @@
+"""GPU parity of the HIP engine against the CPU oracle, through the C ABI.
+
+Bars (stated per check):
+  * delay table: offsets and validity bit-exact; a, b, fd bit-exact (FP64,
+    same operation order, no contraction on either side)
+  * FP32 spectra: per (FFT, antenna, pol) ||dY|| / ||Y|| <= 1e-5 against the
+    FP32 restatement of the reference's IPP path
+  * FP32 visibilities: per (baseline, product) ||dV|| / ||V|| <= 1e-5, counts
+    exact, autocorrelations likewise
+  * FP16 visibilities: per (baseline, product) ||dV|| / ||V64|| <= 1e-3
+    against the FP64 oracle
+  * FP8 visibilities: per (baseline, product) ||dV|| / ||V64|| <= 0.1 for
+    crosses and <= 0.02 for autos against the FP64 oracle (e4m3 keeps 3
+    mantissa bits; the measured error is reported by the benchmark)
+"""
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from difx_tests_amd import FxEngine, synth
+
+pytestmark = pytest.mark.gpu
+
+LO = 1.65e9
+
+
+def _case(nchan, nant, nf, seed, bw=32e6, rate=3e-11, spread=1.4, data="random"):
+    """Random model with net delays of up to +-spread samples (odd offsets,
+    edge clamps and invalid antennas all occur)."""
+    p = orc.plan(nchan, LO, bw)
+    F = 2 * nchan
+    rng = np.random.default_rng(seed)
+    delays = np.stack([rng.uniform(-1e-16, 1e-16, nant), rng.uniform(-rate, rate, nant),
+                       rng.uniform(-2e-6, 2e-6, nant)], axis=1)
+    fileoff = np.round(delays[:, 2] / p.Ts) * p.Ts + rng.uniform(-spread, spread, nant) * p.Ts
+    if data == "random":
+        d = synth.random_bytes(nant, (nf + 1) * F // 2, seed)
+    else:
+        d = synth.make_data(nant, (nf + 1) * F, seed)
+    return p, d, delays, fileoff, np.arange(nf, dtype=np.int32)
+
+
+def _engine(p, nant, nf, prec="f32", data=None):
+    e = FxEngine(nant, p.nchan, LO, p.bw, prec=prec, max_ffts=nf,
+                 max_bytes_per_ant=0 if data is None else data.shape[1])
+    return e
+
+
+def _rel(a, b, axis=-1):
+    return np.linalg.norm(a - b, axis=axis) / np.maximum(np.linalg.norm(b, axis=axis), 1e-30)
+
+
+@pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096])
+def test_delay_table_bit_exact(nchan):
+    nant, nf = 5, 12
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 10 + nchan, rate=2e-10, spread=2.2)
+    with _engine(p, nant, nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.fstage(it)
+        tab = e.read_delay_table(nf)
+    ref = orc.process_f32(p, data, delays, fileoff, it)
+    assert np.array_equal(tab["valid"], ref["valid"])
+    assert np.array_equal(tab["offset"], ref["offset"])
+    assert np.array_equal(tab["fd"], ref["fracdelay"])
+    for k in range(nf):
+        for j in range(nant):
+            m, a, b = orc.station_delay(delays[j], int(it[k]))
+            assert tab["a"][k, j] == a and tab["b"][k, j] == b
+    assert 0 < ref["valid"].sum() < nf * nant          # the case exercises invalid antennas
+
+
+@pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096])
+def test_fstage_fp32_matches_oracle(nchan):
+    nant, nf = 3, 6
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 20 + nchan)
+    with _engine(p, nant, nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.enable_tap(True)
+        e.fstage(it)
+        chan = e.read_tap(nf)
+    ref = orc.process_f32(p, data, delays, fileoff, it, want_chan=True)
+    v = ref["valid"].astype(bool)
+    err = _rel(chan[v], ref["chan"][v])
+    assert err.max() <= 1e-5, err.max()
+    assert not np.any(chan[~v])                          # invalid slots are zero
+
+
+@pytest.mark.parametrize("nchan,nant,nf", [(256, 4, 16), (1024, 4, 9), (2048, 6, 5), (4096, 3, 4), (512, 17, 6)])
+def test_vis_fp32_matches_oracle(nchan, nant, nf):
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 30 + nchan + nant)
+    with _engine(p, nant, nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f32(p, data, delays, fileoff, it)
+    assert np.array_equal(cc, ref["count"]) and np.array_equal(ac, ref["autocount"])
+    live = ref["count"] > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= 1e-5
+    assert not np.any(cross[~live])
+    alive = ref["autocount"] > 0
+    assert _rel(autos[alive], ref["autos"][alive]).max() <= 1e-5
+
+
+def test_vis_fp32_split_and_accumulation():
+    # two calls with split = 2 (kernel.c:730 maxoffset), accumulated on the device
+    nchan, nant, nf = 512, 4, 8
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 77, spread=1.9)
+    half = nf // 2
+    ref = orc.process_f32(p, data, delays, fileoff, it[:half], split=2)
+    ref2 = orc.process_f32(p, data, delays, fileoff, it[half:], split=2)
+    with _engine(p, nant, nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it[:half], split=2)
+        e.process(it[half:], split=2)
+        cross, cc, autos, ac = e.read_vis()
+    assert np.array_equal(cc, ref["count"] + ref2["count"])
+    want = ref["vis"] + ref2["vis"]
+    live = cc > 0
+    assert _rel(cross[live], want[live]).max() <= 1e-5
+
+
+def test_x_batching_over_slots():
+    # two F-stage calls into different FFT slots, one X-engine pass over both
+    nchan, nant, nf = 1024, 4, 8
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 88)
+    with _engine(p, nant, 2 * nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        n0 = e.fstage(it[:4])
+        n1 = e.fstage(it[4:], fft0=n0)
+        e.xstage(0, n0 + n1)
+        cross, cc, autos, ac = e.read_vis()
+    # each call has its own maxoffset: (4 - 1) * F
+    r0 = orc.process_f32(p, data, delays, fileoff, it[:4])
+    r1 = orc.process_f32(p, data, delays, fileoff, it[4:])
+    assert np.array_equal(cc, r0["count"] + r1["count"])
+    live = cc > 0
+    assert _rel(cross[live], (r0["vis"] + r1["vis"])[live]).max() <= 1e-5
+
+
+def test_invalid_antenna_contributes_nothing():
+    nchan, nant, nf = 256, 4, 6
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 99, spread=0.3)
+    fileoff[2] += 1e-3                                   # antenna 2 far outside its window
+    with _engine(p, nant, nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f32(p, data, delays, fileoff, it)
+    assert ref["valid"][:, 2].sum() == 0 and ac[2] == 0 and not np.any(autos[2])
+    assert np.array_equal(cc, ref["count"])
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= 1e-5
+
+
+@pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 32), (4096, 8, 16), (512, 20, 24)])
+def test_vis_fp16_vs_fp64_oracle(nchan, nant, nf):
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 40 + nchan, data="correlated")
+    with _engine(p, nant, nf, prec="f16", data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f64(p, data, delays, fileoff, it)
+    assert np.array_equal(cc, ref["count"])
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= 1e-3
+    assert _rel(autos, ref["autos"]).max() <= 1e-3
+
+
+@pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 64), (2048, 12, 32)])
+def test_vis_fp8_vs_fp64_oracle(nchan, nant, nf):
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 50 + nchan, data="correlated")
+    with _engine(p, nant, nf, prec="f8", data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f64(p, data, delays, fileoff, it)
+    assert np.array_equal(cc, ref["count"])
+    live = cc > 0
+    ec = _rel(cross[live], ref["vis"][live]).max()
+    ea = _rel(autos, ref["autos"]).max()
+    print(f"fp8 max rel error: cross {ec:.3e} auto {ea:.3e}")
+    assert ec <= 0.1 and ea <= 0.02
