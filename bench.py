@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X FX correlator on one GPU.
+
+Workload (default): 64 antennas x 2 polarisations, 2-bit real samples at
+128 Msps, 4096 channels (8192-point FFT), 10 ms subintegrations of 156 FFTs,
+FP16 spectra into the MFMA X-engine.  One step = the full hot path over one
+batch of `--xbatch` subintegrations already resident in HBM: FP64 delay
+table, fused unpack/fringe/FFT/fractional-delay F-stage (one launch per
+subint), valid-FFT counts, and one X-engine launch accumulating the batch
+into the visibilities.  The data are synthetic (seeded 2-bit Gaussian noise
+with a 0.1-correlated common signal) and every subint of a batch reuses the
+same buffer.
+
+Prints one JSON line.  `value` is input Gsamples/s (antennas x pols x
+samples processed per second).  The same line carries the X-engine TFLOPS,
+the plain FP32 GPU pipeline on the same workload, the roofline of the
+dominant kernel (HIP-event timings on the engine's stream inside the timed
+region) and a bounded single-thread CPU baseline (the oracle's FP32
+restatement of the reference path).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from difx_tests_amd import FxEngine, workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f16": 2500.0, "f8": 5000.0, "f32": 157.3}
+OUT_BYTES_PER_SAMPLE = {"f32": 4.0, "f16": 2.0, "f8": 1.0}   # nchan complex bins per F real samples
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c3-64ant-4096ch", choices=sorted(workloads.WORKLOADS))
+    ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8"])
+    ap.add_argument("--xbatch", type=int, default=1, help="subints per X-engine launch")
+    ap.add_argument("--no-fp32-compare", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work (s); 0 disables")
+    ap.add_argument("--seed", type=int, default=20261205)
+    return ap.parse_args()
+
+
+def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch):
+    """Returns (seconds per step, kernel ms per step dict, launches)."""
+    nf = len(it)
+    e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec=prec, max_ffts=xbatch * (nf + 64),
+                 max_bytes_per_ant=data.shape[1])
+    try:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        gran = e.granule
+        slots = (nf + gran - 1) // gran * gran
+
+        def step():
+            for s in range(xbatch):
+                e.fstage(it, 1, s * slots)
+            e.xstage(0, xbatch * slots)
+
+        e.reset()
+        for _ in range(warmup):
+            step()
+        e.sync()
+        e.set_timing(True)
+        e.reset_timing()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        e.sync()
+        t1 = time.perf_counter()
+        ms, nl = e.timing()
+        e.set_timing(False)
+        per_step = {k: v / steps for k, v in ms.items()}
+        return (t1 - t0) / steps, per_step, nl, slots
+    finally:
+        e.close()
+
+
+def cpu_baseline(w, data, delays, fileoff, budget_s):
+    """Single-thread FP32 restatement (oracle) on a bounded FFT sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as orc
+    p = orc.plan(w.nchan, w.lo, w.bw)
+    # probe with one FFT of every antenna, then size the sample to the budget
+    t0 = time.perf_counter()
+    chan, valid = orc.fstage_f32(p, data, delays, fileoff, np.arange(1, dtype=np.int32), split=w.nffts)
+    orc.xstage_f32(chan, valid)
+    t1 = time.perf_counter()
+    nsample = int(max(1, min(w.nffts, budget_s / max(t1 - t0, 1e-6))))
+    it = np.arange(nsample, dtype=np.int32)
+    t0 = time.perf_counter()
+    chan, valid = orc.fstage_f32(p, data, delays, fileoff, it, split=1 if nsample == w.nffts else w.nffts // nsample,
+                                 nthreads=1)
+    t1 = time.perf_counter()
+    orc.xstage_f32(chan, valid, nthreads=1)
+    t2 = time.perf_counter()
+    samples = w.nant * 2 * nsample * w.F
+    return {
+        "value": round(samples / (t2 - t0) / 1e9, 6),
+        "unit": "Gsamples/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{nsample} FFTs x {w.nant} antennas x 2 pols of the same subint "
+                  f"(F {t1 - t0:.2f} s + X {t2 - t1:.2f} s, single thread, oracle FP32 restatement, no IPP)",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note, not implemented)"}))
+        sys.exit(2)
+    w = workloads.WORKLOADS[a.workload]
+    prec = a.prec or w.prec
+    data, delays, fileoff, it = workloads.make_subint(w, a.seed)
+    nf = len(it)
+    samples_per_step = a.xbatch * w.samples_per_subint
+
+    sec, kms, nl, slots = run_pipeline(w, prec, data, delays, fileoff, it, a.steps, a.warmup, a.xbatch)
+    value = samples_per_step / sec / 1e9
+
+    # algorithmic work per launch
+    f_launch_bytes = w.samples_per_subint * (0.25 + OUT_BYTES_PER_SAMPLE[prec])
+    x_flops = 8.0 * 4 * w.nbl_with_autos * w.nchan * nf * a.xbatch
+    f_ms = kms["fstage"] / a.xbatch          # per F launch
+    x_ms = kms["xstage"]                      # one X launch per step
+    f_gbs = f_launch_bytes / (f_ms * 1e-3) / 1e9
+    x_tflops = x_flops / (x_ms * 1e-3) / 1e12
+    if kms["fstage"] >= kms["xstage"]:
+        roof = {"kernel": "fx_fstage_kernel", "bound": "hbm", "achieved": round(f_gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(f_ms, 4)}
+    else:
+        pk = MFMA_PEAK_TFLOPS[prec]
+        roof = {"kernel": "fx_xengine_kernel", "bound": "mfma", "achieved": round(x_tflops, 2), "peak": pk,
+                "unit": "TFLOP/s", "frac": round(x_tflops / pk, 4), "traffic": None, "avg_launch_ms": round(x_ms, 4)}
+
+    out = {
+        "metric": "input Gsamples/s/GPU + X-engine TFLOPS; FP16-MFMA vs FP32 at <=1e-3 rel err",
+        "value": round(value, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic",
+        "config": {"workload": w.name, "antennas": w.nant, "pols": 2, "nchan": w.nchan, "fs_msps": w.fs / 1e6,
+                   "subint_ms": w.subint_s * 1e3, "ffts_per_subint": nf, "subints_per_step": a.xbatch,
+                   "xengine_prec": prec, "parallelism": "single GPU"},
+        "xengine_tflops": round(x_tflops, 2),
+        "fstage_gbs": round(f_gbs, 1),
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in kms.items()},
+        "realtime_factor": round((a.xbatch * w.subint_s) / sec, 2),
+        "roofline": roof,
+    }
+    if not a.no_fp32_compare and prec != "f32":
+        s32, k32, _, _ = run_pipeline(w, "f32", data, delays, fileoff, it, max(3, a.steps // 2), 1, a.xbatch)
+        v32 = samples_per_step / s32 / 1e9
+        out["fp32_pipeline"] = {"value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
+                                "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
+                                "speedup_vs_fp32": round(value / v32, 3)}
+    if a.cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline(w, data, delays, fileoff, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

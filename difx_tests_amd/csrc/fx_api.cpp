@@ -33,7 +33,13 @@ static int fail(int code, const char* fmt, ...)
                         __FILE__, __LINE__);                                              \
     } while (0)
 
-enum { EV_D0, EV_D1, EV_F1, EV_C1, EV_X0, EV_X1, EV_N };
+// Kernel timing: event pairs recorded around launches on the context's
+// stream without synchronising; fx_get_timing waits and sums them.
+enum { TM_DELAY, TM_FSTAGE, TM_COUNTS, TM_XSTAGE, TM_N };
+struct EvPair {
+    hipEvent_t a, b;
+    int kind;
+};
 
 struct fx_context {
     FxPlan p;
@@ -67,8 +73,9 @@ struct fx_context {
     int tap;
     int last_nffts;     // entries in the delay table of the last fx_fstage
     int timing;
-    hipEvent_t ev[EV_N];
-    double ms[4];
+    std::vector<EvPair> pool;      // free event pairs
+    std::vector<EvPair> pending;   // recorded, not yet summed
+    double ms[TM_N];
     int nl[2];
 };
 
@@ -89,8 +96,11 @@ static void free_ctx(fx_context* c)
                     c->d_cross, c->d_autos, c->d_tap};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    for (int i = 0; i < EV_N; i++)
-        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (auto* v : {&c->pool, &c->pending})
+        for (EvPair& e : *v) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -143,7 +153,6 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(FX_EHIP, "no HIP device available");
     fx_context* c = new fx_context();
-    memset(c->ev, 0, sizeof(c->ev));
     int rc = make_plan(prm, &c->p);
     if (rc) {
         delete c;
@@ -173,11 +182,6 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
         free_ctx(c);
         return fail(FX_EHIP, "hipStreamCreate failed");
     }
-    for (int i = 0; i < EV_N; i++)
-        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
-            free_ctx(c);
-            return fail(FX_EHIP, "hipEventCreate failed");
-        }
     ALLOC(c->d_data, (size_t)c->stride * p.nant);
     ALLOC(c->d_model, sizeof(double) * 4 * p.nant);
     ALLOC(c->d_iter, sizeof(int) * c->max_ffts);
@@ -287,11 +291,55 @@ extern "C" int fx_device_data(fx_context* c, void** dptr, long long* stride)
     return FX_OK;
 }
 
-static void acc_time(fx_context* c, int slot, hipEvent_t a, hipEvent_t b)
+static int ev_begin(fx_context* c, int kind, EvPair** out)
 {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, a, b) == hipSuccess) c->ms[slot] += ms;
+    *out = nullptr;
+    if (!c->timing) return FX_OK;
+    EvPair e;
+    if (!c->pool.empty()) {
+        e = c->pool.back();
+        c->pool.pop_back();
+    } else {
+        HIPCHK(hipEventCreate(&e.a));
+        HIPCHK(hipEventCreate(&e.b));
+    }
+    e.kind = kind;
+    HIPCHK(hipEventRecord(e.a, c->stream));
+    c->pending.push_back(e);
+    *out = &c->pending.back();
+    return FX_OK;
 }
+
+static int ev_end(fx_context* c)
+{
+    if (!c->timing) return FX_OK;
+    HIPCHK(hipEventRecord(c->pending.back().b, c->stream));
+    return FX_OK;
+}
+
+static int ev_collect(fx_context* c)
+{
+    if (c->pending.empty()) return FX_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (EvPair& e : c->pending) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+        c->ms[e.kind] += ms;
+        c->pool.push_back(e);
+    }
+    c->pending.clear();
+    return FX_OK;
+}
+
+#define TIMED(kind, launch)                      \
+    do {                                         \
+        EvPair* ep_;                             \
+        int rc_ = ev_begin(c, (kind), &ep_);     \
+        if (rc_) return rc_;                     \
+        HIPCHK(launch);                          \
+        rc_ = ev_end(c);                         \
+        if (rc_) return rc_;                     \
+    } while (0)
 
 extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int split, int fft0, int* nslots)
 {
@@ -316,10 +364,8 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
         if (hipMalloc((void**)&c->d_tap, sizeof(float2) * (size_t)c->max_ffts * p.nant * 2 * p.nchan) != hipSuccess)
             return fail(FX_ENOMEM, "tap allocation failed");
     }
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_D0], c->stream));
-    HIPCHK(launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, numffts, npad, (int)maxoffset, c->d_dtab,
-                        c->stream));
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_D1], c->stream));
+    TIMED(TM_DELAY, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, numffts, npad, (int)maxoffset,
+                                 c->d_dtab, c->stream));
     FStageArgs a;
     a.p = p;
     a.data = c->d_data;
@@ -334,17 +380,9 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
     a.ngroups = npad / c->G;
     a.scale_inv = c->d_scale_inv;
     a.tap = c->tap ? c->d_tap : nullptr;
-    HIPCHK(launch_fstage(a, c->prec, c->stream));
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_F1], c->stream));
-    HIPCHK(launch_counts(p.nant, npad, c->d_dtab, c->d_counts, c->stream));
-    if (c->timing) {
-        HIPCHK(hipEventRecord(c->ev[EV_C1], c->stream));
-        HIPCHK(hipEventSynchronize(c->ev[EV_C1]));
-        acc_time(c, 0, c->ev[EV_D0], c->ev[EV_D1]);
-        acc_time(c, 1, c->ev[EV_D1], c->ev[EV_F1]);
-        acc_time(c, 2, c->ev[EV_F1], c->ev[EV_C1]);
-        c->nl[0]++;
-    }
+    TIMED(TM_FSTAGE, launch_fstage(a, c->prec, c->stream));
+    TIMED(TM_COUNTS, launch_counts(p.nant, npad, c->d_dtab, c->d_counts, c->stream));
+    if (c->timing) c->nl[0]++;
     c->last_nffts = npad;
     if (nslots) *nslots = npad;
     return FX_OK;
@@ -369,14 +407,8 @@ extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
     a.ngroups_wg = c->waves;
     a.accumulate = 1;
     a.scale = c->d_scale;
-    if (c->timing) HIPCHK(hipEventRecord(c->ev[EV_X0], c->stream));
-    HIPCHK(launch_xengine(a, c->prec, c->stream));
-    if (c->timing) {
-        HIPCHK(hipEventRecord(c->ev[EV_X1], c->stream));
-        HIPCHK(hipEventSynchronize(c->ev[EV_X1]));
-        acc_time(c, 3, c->ev[EV_X0], c->ev[EV_X1]);
-        c->nl[1]++;
-    }
+    TIMED(TM_XSTAGE, launch_xengine(a, c->prec, c->stream));
+    if (c->timing) c->nl[1]++;
     return FX_OK;
 }
 
@@ -484,6 +516,8 @@ extern "C" int fx_set_timing(fx_context* c, int enable)
 extern "C" int fx_reset_timing(fx_context* c)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
+    int rc = ev_collect(c);
+    if (rc) return rc;
     for (double& v : c->ms) v = 0.0;
     c->nl[0] = c->nl[1] = 0;
     return FX_OK;
@@ -492,6 +526,8 @@ extern "C" int fx_reset_timing(fx_context* c)
 extern "C" int fx_get_timing(fx_context* c, double* ms4, int* nlaunch2)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
+    int rc = ev_collect(c);
+    if (rc) return rc;
     if (ms4)
         for (int i = 0; i < 4; i++) ms4[i] = c->ms[i];
     if (nlaunch2) {

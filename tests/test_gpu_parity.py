@@ -10,8 +10,10 @@ Bars (stated per check):
   * FP16 visibilities: per (baseline, product) ||dV|| / ||V64|| <= 1e-3
     against the FP64 oracle
   * FP8 visibilities: per (baseline, product) ||dV|| / ||V64|| <= 0.1 for
-    crosses and <= 0.02 for autos against the FP64 oracle (e4m3 keeps 3
-    mantissa bits; the measured error is reported by the benchmark)
+    crosses and cross-hand autocorrelations, <= 0.02 for parallel-hand
+    autocorrelations, against the FP64 oracle.  e4m3 keeps 3 mantissa bits,
+    so each product carries ~5% rms rounding error; on noise-dominated
+    products that error does not average down relative to the visibility.
 """
 import numpy as np
 import pytest
@@ -32,7 +34,14 @@ def _case(nchan, nant, nf, seed, bw=32e6, rate=3e-11, spread=1.4, data="random")
     rng = np.random.default_rng(seed)
     delays = np.stack([rng.uniform(-1e-16, 1e-16, nant), rng.uniform(-rate, rate, nant),
                        rng.uniform(-2e-6, 2e-6, nant)], axis=1)
-    fileoff = np.round(delays[:, 2] / p.Ts) * p.Ts + rng.uniform(-spread, spread, nant) * p.Ts
+    perturb = rng.uniform(-spread, spread, nant)
+    if spread >= 2.0:
+        # net delays that hit the edge cases: invalid at the first FFT, invalid
+        # at the last FFT, the o = -1 and o = maxoffset + 1 clamps (kernel.c:742-755)
+        edge = [-2.0, 2.6, -1.0, 1.6, 0.49]
+        perturb[: min(nant, len(edge))] = edge[: min(nant, len(edge))]
+    # net delay at FFT 0 is -perturb samples (plus the small rate term)
+    fileoff = delays[:, 2] + perturb * p.Ts
     if data == "random":
         d = synth.random_bytes(nant, (nf + 1) * F // 2, seed)
     else:
@@ -191,6 +200,7 @@ def test_vis_fp8_vs_fp64_oracle(nchan, nant, nf):
     assert np.array_equal(cc, ref["count"])
     live = cc > 0
     ec = _rel(cross[live], ref["vis"][live]).max()
-    ea = _rel(autos, ref["autos"]).max()
-    print(f"fp8 max rel error: cross {ec:.3e} auto {ea:.3e}")
-    assert ec <= 0.1 and ea <= 0.02
+    ea = _rel(autos[:, [0, 3]], ref["autos"][:, [0, 3]]).max()
+    ex = _rel(autos[:, [1, 2]], ref["autos"][:, [1, 2]]).max()
+    print(f"fp8 max rel error: cross {ec:.3e} parallel-hand auto {ea:.3e} cross-hand auto {ex:.3e}")
+    assert ec <= 0.1 and ex <= 0.1 and ea <= 0.02
