@@ -34,8 +34,8 @@ sys.path.insert(0, ROOT)
 from difx_tests_amd import FxEngine, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"f16": 2500.0, "f8": 5000.0, "f32": 157.3}
-OUT_BYTES_PER_SAMPLE = {"f32": 4.0, "f16": 2.0, "f8": 1.0}   # nchan complex bins per F real samples
+MFMA_PEAK_TFLOPS = {"f16": 2500.0, "f8": 5000.0, "f32": 157.3, "f32v": 157.3}
+OUT_BYTES_PER_SAMPLE = {"f32": 4.0, "f16": 2.0, "f8": 1.0, "f32v": 4.0}   # nchan complex bins per F real samples
 
 
 def parse():
@@ -44,8 +44,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3-64ant-4096ch", choices=sorted(workloads.WORKLOADS))
-    ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8"])
-    ap.add_argument("--xbatch", type=int, default=1, help="subints per X-engine launch")
+    ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8", "f32v"])
+    ap.add_argument("--xbatch", type=int, default=8, help="subints per X-engine launch")
     ap.add_argument("--no-fp32-compare", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work (s); 0 disables")
     ap.add_argument("--seed", type=int, default=20261205)
@@ -169,12 +169,14 @@ def main():
         "realtime_factor": round((a.xbatch * w.subint_s) / sec, 2),
         "roofline": roof,
     }
-    if not a.no_fp32_compare and prec != "f32":
-        s32, k32, _, _ = run_pipeline(w, "f32", data, delays, fileoff, it, max(3, a.steps // 2), 1, a.xbatch)
-        v32 = samples_per_step / s32 / 1e9
-        out["fp32_pipeline"] = {"value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
-                                "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
-                                "speedup_vs_fp32": round(value / v32, 3)}
+    if not a.no_fp32_compare and prec not in ("f32", "f32v"):
+        # the plain FP32 GPU pipeline (vector-ALU X-engine) and the FP32 MFMA one
+        for key, p32 in (("fp32_pipeline", "f32v"), ("fp32_mfma_pipeline", "f32")):
+            s32, k32, _, _ = run_pipeline(w, p32, data, delays, fileoff, it, max(3, a.steps // 2), 1, a.xbatch)
+            v32 = samples_per_step / s32 / 1e9
+            out[key] = {"xengine": p32, "value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
+                        "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
+                        "speedup_vs_this": round(value / v32, 3)}
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(w, data, delays, fileoff, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -43,10 +43,11 @@ struct EvPair {
 
 struct fx_context {
     FxPlan p;
-    int prec;
+    int prec;           // X-engine precision (FX_PREC_*)
+    int fprec;          // F-stage output format (FP32 for both FP32 X-engines)
     int G;              // FFTs per corner-turn group
-    int kgroups;        // groups per X-engine K step
-    int granule;        // G * kgroups: FFT slots written per K step
+    int kgroups;        // groups per X-engine K chunk
+    int granule;        // G * kgroups: FFT slots per K chunk
     int nfg;            // groups held by the X input
     int max_ffts;       // padded FFT capacity
     long long cap_bytes, stride;
@@ -63,8 +64,6 @@ struct fx_context {
     float2* d_vacc;
     size_t vacc_bytes;
     int* d_counts;
-    int4* d_tasks;
-    int ntasks, waves;
     float* d_scale;     // [nchan][Spad] (FP8)
     float* d_scale_inv;
     float2* d_cross;
@@ -92,7 +91,7 @@ extern "C" int fx_fft_granule(const fx_context* ctx) { return ctx ? ctx->granule
 static void free_ctx(fx_context* c)
 {
     void* bufs[] = {c->d_data, c->d_model, c->d_iter, c->d_dtab, c->d_tw, c->d_subchf, c->d_stepchf,
-                    c->d_xin, c->d_vacc, c->d_counts, c->d_tasks, c->d_scale, c->d_scale_inv,
+                    c->d_xin, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv,
                     c->d_cross, c->d_autos, c->d_tap};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -147,7 +146,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
 {
     if (!prm || !out) return fail(FX_EINVAL, "null argument");
     *out = nullptr;
-    if (prm->prec < FX_PREC_F32 || prm->prec > FX_PREC_F8) return fail(FX_EINVAL, "bad precision %d", prm->prec);
+    if (prm->prec < FX_PREC_F32 || prm->prec > FX_PREC_F32V) return fail(FX_EINVAL, "bad precision %d", prm->prec);
     if (prm->max_ffts < 1) return fail(FX_EINVAL, "max_ffts must be >= 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -160,8 +159,9 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     }
     FxPlan& p = c->p;
     c->prec = prm->prec;
-    c->G = fx_fstage_group_size(prm->prec);
-    c->kgroups = prm->prec == FX_PREC_F16 ? 2 : prm->prec == FX_PREC_F8 ? 4 : 1;
+    c->fprec = prm->prec == FX_PREC_F32V ? FX_PREC_F32 : prm->prec;   // F-stage output format
+    c->G = fx_fstage_group_size(c->fprec);
+    c->kgroups = xengine_chunk_groups(prm->prec);
     c->granule = c->G * c->kgroups;
     p.G = c->G;
     c->max_ffts = (prm->max_ffts + c->granule - 1) / c->granule * c->granule;
@@ -211,16 +211,6 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     for (int s = 0; s < p.stride; s++) subchf[s] = (float)((2 * PI * s * p.bw) / (double)p.nchan);     // kernel.c:164
     for (int k = 0; k < p.nfstep; k++)
         stepchf[k] = (float)((2 * PI * k * p.stride * p.bw) / (double)p.nchan);                      // kernel.c:165
-    // X-engine wave tasks: row block I, column blocks J0.. (up to 4)
-    std::vector<int4> tasks;
-    for (int I = 0; I < p.nblk; I++)
-        for (int J0 = I; J0 < p.nblk; J0 += 4) {
-            const int nb = (p.nblk - J0) < 4 ? (p.nblk - J0) : 4;
-            tasks.push_back(make_int4(I, J0, nb, fx_bp_index(I, J0, p.nblk)));
-        }
-    c->ntasks = (int)tasks.size();
-    c->waves = c->ntasks < 4 ? c->ntasks : 4;
-    ALLOC(c->d_tasks, sizeof(int4) * tasks.size());
 #undef ALLOC
     // FP8 default scales: the spectrum of 2-bit noise has E|Y|^2 = F * E[L^2]
     // with E[L^2] = 4.3128 for levels {+-1, +-3.3359} at +-0.98 sigma; map one
@@ -231,7 +221,6 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * p.F, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_subchf, subchf.data(), sizeof(float) * p.stride, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_stepchf, stepchf.data(), sizeof(float) * p.nfstep, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_tasks, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_scale, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_scale_inv, sci.data(), sizeof(float) * sci.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->d_data, 0, (size_t)c->stride * p.nant) != hipSuccess ||
@@ -380,7 +369,7 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
     a.ngroups = npad / c->G;
     a.scale_inv = c->d_scale_inv;
     a.tap = c->tap ? c->d_tap : nullptr;
-    TIMED(TM_FSTAGE, launch_fstage(a, c->prec, c->stream));
+    TIMED(TM_FSTAGE, launch_fstage(a, c->fprec, c->stream));
     TIMED(TM_COUNTS, launch_counts(p.nant, npad, c->d_dtab, c->d_counts, c->stream));
     if (c->timing) c->nl[0]++;
     c->last_nffts = npad;
@@ -402,9 +391,6 @@ extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
     a.fg0 = fft0 / c->G;
     a.ngroups = nslots / c->G;
     a.vacc = c->d_vacc;
-    a.tasks = c->d_tasks;
-    a.ntasks = c->ntasks;
-    a.ngroups_wg = c->waves;
     a.accumulate = 1;
     a.scale = c->d_scale;
     TIMED(TM_XSTAGE, launch_xengine(a, c->prec, c->stream));

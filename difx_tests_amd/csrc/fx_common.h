@@ -21,6 +21,7 @@
 #define FX_PREC_F32 0
 #define FX_PREC_F16 1
 #define FX_PREC_F8 2
+#define FX_PREC_F32V 3
 #endif
 
 struct FxDelay {           // 32 bytes; one per (FFT, antenna)

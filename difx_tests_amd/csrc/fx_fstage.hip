@@ -203,7 +203,7 @@ __device__ __forceinline__ void dft(float2 (&x)[R])
 template <int R>
 __device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __restrict__ tw, int e)
 {
-    float2 lo[4], hi[R / 4];
+    float2 lo[4], hi[(R / 4) > 0 ? R / 4 : 1];
     lo[0] = make_float2(1.0f, 0.0f);
 #pragma unroll
     for (int b = 1; b < 4 && b < R; b++) lo[b] = tw[b * e];
@@ -222,18 +222,24 @@ __device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __rest
 }
 
 // --------------------------------------------------------------------------
-// Compile-time geometry for F = 2^LG.  Passes R0 (fused unpack), R1, R2 (last,
-// lower half only); threads NT = F / R0 so pass 0 gives every thread one
-// butterfly of both polarisations.
+// Radix plans for F = 2^LG.  Pass 0 is fused with unpack and fringe
+// rotation, the last pass forms only channels < nchan and applies the
+// fractional-sample rotation.  A block transforms one polarisation and has
+// NT = F/R_max threads, so a radix-16 pass hands each thread exactly one
+// butterfly; at F = 8192 two blocks (79 KB of LDS each) share a CU, so one
+// block's barriers and window loads overlap the other's arithmetic.
 
-template <int LG> struct RPlan;
-template <> struct RPlan<9>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8; };
-template <> struct RPlan<10> { static constexpr int R0 = 16, R1 = 8,  R2 = 8; };
-template <> struct RPlan<11> { static constexpr int R0 = 16, R1 = 16, R2 = 8; };
-template <> struct RPlan<12> { static constexpr int R0 = 16, R1 = 16, R2 = 16; };
-template <> struct RPlan<13> { static constexpr int R0 = 16, R1 = 16, R2 = 32; };
+template <int LG> struct FPlan;
+template <> struct FPlan<9>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1; };
+template <> struct FPlan<10> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1; };
+template <> struct FPlan<11> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1; };
+template <> struct FPlan<12> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
+template <> struct FPlan<13> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2; };
+
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <int LG> struct Geo {
+    using PL_ = FPlan<LG>;
     static constexpr int F = 1 << LG;
     static constexpr int NCH = F / 2;
     static constexpr int STRIDE = 1 << ((LG - 1) / 2);     // 2^floor(log2(nchan)/2)
@@ -241,10 +247,19 @@ template <int LG> struct Geo {
     static constexpr int NSTEP = F / SUB;
     static constexpr int NFSTEP = NCH / STRIDE;
     static constexpr int NTAB = SUB + NSTEP + STRIDE + NFSTEP;
-    static constexpr int R0 = RPlan<LG>::R0, R1 = RPlan<LG>::R1, R2 = RPlan<LG>::R2;
-    static constexpr int NT = F / R0;
-    static_assert(R0 * R1 * R2 == F, "plan must factor F");
-    static_assert(NT % SUB == 0, "pass-0 rotator reuse needs F/R0 to be a multiple of sub");
+    static constexpr int NP = PL_::NP, R0 = PL_::R0, R1 = PL_::R1, R2 = PL_::R2, R3 = PL_::R3;
+    static constexpr int RMAX = cmax(cmax(R0, R1), cmax(R2, R3));
+    static constexpr int NT = F / RMAX;
+    static constexpr int RL = NP == 4 ? R3 : R2;            // last radix
+    static constexpr int PL = F / RL;                       // transform length before the last pass
+    static constexpr int NOUT = NCH / NT;                   // channels per thread per FFT
+    static constexpr int FP = F + F / 16;                   // padded LDS row
+    // waves per SIMD the LDS allows at F = 8192 (two 8-wave blocks per CU)
+    static constexpr int WPE = LG >= 13 ? 4 : 3;
+    static_assert(R0 * R1 * R2 * R3 == F, "plan must factor F");
+    static_assert((F / R0) % SUB == 0, "pass-0 rotator reuse needs F/R0 to be a multiple of sub");
+    static_assert(R0 == 16 || (R0 == 8 && F / R0 >= 64), "pass-0 writes pad(i*R0 + r) = pad(i*R0) + r");
+    static_assert(NT <= 1024, "block too large");
 };
 
 __device__ __forceinline__ int pad(int n) { return n + (n >> 4); }  // LDS bank skew
@@ -317,49 +332,101 @@ int fx_fstage_group_size(int prec)
 }
 
 // --------------------------------------------------------------------------
-// The fused F-stage kernel.
+// Middle pass: radix R on sub-transforms of length P (Stockham, in place in
+// LDS with a barrier between the reads and the writes).
 
-template <int LG, int PREC, bool TAP>
-__global__ __launch_bounds__(Geo<LG>::NT) void fx_fstage_kernel(FStageArgs args)
+template <int LG, int R, int P, bool TW_LDS>
+__device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const float2* __restrict__ tw, int tid)
 {
     using Gm = Geo<LG>;
-    constexpr int F = Gm::F, NCH = Gm::NCH, NT = Gm::NT;
-    constexpr int STRIDE = Gm::STRIDE, SUB = Gm::SUB, NSTEP = Gm::NSTEP, NFSTEP = Gm::NFSTEP;
-    constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2;
-    constexpr int G = GroupSize<PREC>::G;
-    constexpr int T0 = F / R0, T1 = F / R1, T2 = F / R2;   // butterflies per pol per pass
-    constexpr int P1 = R0, P2 = R0 * R1;                    // sub-transform lengths
-    constexpr int PER1 = 2 * T1 / NT, PER2 = 2 * T2 / NT;   // (pol, butterfly) units per thread
-    static_assert(PER1 * NT == 2 * T1 && PER2 * NT == 2 * T2, "plan/thread mismatch");
-    static_assert(T0 == NT, "pass 0 maps one butterfly to each thread");
-    constexpr int NOUT = PER2 * (R2 / 2);
-    constexpr int FP = F + F / 16;
+    constexpr int F = Gm::F, NT = Gm::NT, FP = Gm::FP;
+    constexpr int T = F / R, PER = T / NT;
+    static_assert(PER * NT == T, "plan/thread mismatch");
+    static_assert(T % 16 == 0, "padded addressing needs T to be a multiple of 16");
+    float2 x[PER][R];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int i = tid + m * NT;
+        const int k = i & (P - 1);
+        // T and P are multiples of 16, so pad(i + r*T) = pad(i) + r*(T + T/16):
+        // one base address and immediate offsets per butterfly
+        const float2* rd = buf + pad(i);
+#pragma unroll
+        for (int r = 0; r < R; r++) x[m][r] = rd[r * (T + T / 16)];
+        if (TW_LDS) {
+#pragma unroll
+            for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tw1[r * k]);
+        } else {
+            twiddle_pow<R>(x[m], tw, k * (F / (P * R)));
+        }
+        dft<R>(x[m]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int i = tid + m * NT;
+        const int k = i & (P - 1);
+        const int j = (i - k) * R + k;
+        if constexpr (P % 16 == 0) {
+            float2* wr = buf + pad(j);
+#pragma unroll
+            for (int r = 0; r < R; r++) wr[r * (P + P / 16)] = x[m][r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[pad(j + r * P)] = x[m][r];
+        }
+    }
+    __syncthreads();
+}
 
-    __shared__ __align__(16) float2 buf[2 * FP];
-    __shared__ __align__(16) float4 lut[256];
+// --------------------------------------------------------------------------
+// The fused F-stage kernel: one block = one antenna, one polarisation, G
+// consecutive FFTs.
+
+template <int LG, int PREC, bool TAP>
+__global__ __launch_bounds__(Geo<LG>::NT) __attribute__((amdgpu_waves_per_eu(Geo<LG>::WPE)))
+void fx_fstage_kernel(FStageArgs args)
+{
+    using Gm = Geo<LG>;
+    constexpr int F = Gm::F, NCH = Gm::NCH, NT = Gm::NT, FP = Gm::FP;
+    constexpr int STRIDE = Gm::STRIDE, SUB = Gm::SUB, NSTEP = Gm::NSTEP, NFSTEP = Gm::NFSTEP;
+    constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2, RL = Gm::RL, PL = Gm::PL;
+    constexpr int G = GroupSize<PREC>::G;
+    constexpr int T0 = F / R0, PER0 = T0 / NT;
+    constexpr int TL = F / RL, PERL = TL / NT;
+    static_assert(PER0 * NT == T0 && PERL * NT == TL, "plan/thread mismatch");
+    constexpr int NOUT = PERL * (RL / 2);
+    static_assert(NOUT == Gm::NOUT, "output count");
+    constexpr int NTW1 = R0 * R1;
+
+    __shared__ __align__(16) float2 buf[FP];
+    __shared__ __align__(16) float2 lut[256];           // byte -> (level t0, level t1) of this pol
     __shared__ __align__(16) uint32_t bytes[F / 8 + 4];
     __shared__ float2 rsub[SUB], rstep[NSTEP], frsub[STRIDE], frstep[NFSTEP];
-    __shared__ float2 tw1[P1 * R1];                 // pass-1 twiddles exp(-2 pi i m / (P1 R1))
+    __shared__ float2 tw1[NTW1];                         // exp(-2 pi i m / (R0 R1))
 
     const FxPlan& p = args.p;
-    const int tid = threadIdx.x;
 
     // XCD-aware order: hardware places block b on XCD b % 8; give each XCD a
-    // contiguous range of (group, antenna) work so the G-FFT runs written by
-    // neighbouring antennas meet in the same L2.
+    // contiguous range of (group, antenna, pol) work so the G-FFT runs written
+    // by neighbouring antennas meet in the same L2.
     const int nblocks = gridDim.x;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
     const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
     const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-    const int grp = lid / p.nant;
-    const int ant = lid - grp * p.nant;
+    const int pol = lid & 1;
+    const int grp = (lid >> 1) / p.nant;
+    const int ant = (lid >> 1) - grp * p.nant;
     const int fgo = args.fg0 + grp;
+    const int sig = 2 * ant + pol;
+    int tid = threadIdx.x;
 
-    for (int e = tid; e < P1 * R1; e += NT) tw1[e] = args.tw[e * (F / (P1 * R1))];
+    for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
     for (int e = tid; e < 256; e += NT) {
+        // pol p of time slot t sits at bits 2*(2t+p) (kernel.c:195-200)
         const float hi = 3.3359f;
         const float lv[4] = {-hi, -1.0f, 1.0f, hi};
-        lut[e] = make_float4(lv[e & 3], lv[(e >> 2) & 3], lv[(e >> 4) & 3], lv[(e >> 6) & 3]);
+        lut[e] = make_float2(lv[(e >> (2 * pol)) & 3], lv[(e >> (4 + 2 * pol)) & 3]);
     }
 
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
@@ -367,6 +434,10 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fstage_kernel(FStageArgs args)
 
 #pragma unroll 1
     for (int gi = 0; gi < G; gi++) {
+        // Redefine tid for the compiler on every FFT: the LDS addresses derived
+        // from it are then formed where they are used instead of being hoisted
+        // out of the loop and held in registers across all passes.
+        asm volatile("" : "+v"(tid));
         outr.shift();
         const int fft = grp * G + gi;             // FFT of this launch
         const FxDelay d = args.delays[(long long)fft * p.nant + ant];
@@ -375,14 +446,11 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fstage_kernel(FStageArgs args)
             outr.zero_last();
             if (TAP) {
 #pragma unroll
-                for (int m = 0; m < PER2; m++) {
-                    const int u = tid + m * NT;
-                    const int pol = u / T2, i = u % T2;
+                for (int m = 0; m < PERL; m++)
 #pragma unroll
-                    for (int r = 0; r < R2 / 2; r++)
-                        args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + i + r * P2] =
+                    for (int r = 0; r < RL / 2; r++)
+                        args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + tid + m * NT + r * PL] =
                             make_float2(0.0f, 0.0f);
-                }
             }
             continue;
         }
@@ -432,93 +500,73 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fstage_kernel(FStageArgs args)
         for (int w = tid; w < nd; w += NT) bytes[w] = src32[w];
         __syncthreads();
 
-        // ---- pass 0: unpack + fringe rotate + radix-R0 butterfly, both pols
+        // ---- pass 0: unpack + fringe rotate + radix-R0 butterfly
         {
             const uint8_t* bl = reinterpret_cast<const uint8_t*>(bytes);
-            const int i = tid;
-            const float2 rs = rsub[i & (SUB - 1)];
-            float2 x0[R0], x1[R0];
 #pragma unroll
-            for (int r = 0; r < R0; r++) {
-                const int n = i + r * T0;
-                const float2 rot = cmul_exact(rs, rstep[n / SUB]);
-                const int q = o + n;
-                const float4 lv = lut[bl[(q >> 1) - bbase]];
-                const float l0 = (q & 1) ? lv.z : lv.x;
-                const float l1 = (q & 1) ? lv.w : lv.y;
-                x0[r] = make_float2(l0 * rot.x, l0 * rot.y);
-                x1[r] = make_float2(l1 * rot.x, l1 * rot.y);
-            }
-            dft<R0>(x0);
-            dft<R0>(x1);
+            for (int m = 0; m < PER0; m++) {
+                const int i = tid + m * NT;
+                const float2 rs = rsub[i & (SUB - 1)];
+                // sample n = i + r*T0: T0 is even and a multiple of SUB, so the
+                // time slot is fixed per thread and the byte, step-rotator and
+                // LDS write addresses are a base plus r times a constant
+                const int q0 = o + i;
+                const bool odd = q0 & 1;
+                const uint8_t* bp = bl + ((q0 >> 1) - bbase);
+                const float2* sp = rstep + i / SUB;
+                float2 x[R0];
 #pragma unroll
-            for (int r = 0; r < R0; r++) {
-                buf[pad(i * R0 + r)] = x0[r];
-                buf[FP + pad(i * R0 + r)] = x1[r];
-            }
-        }
-        __syncthreads();
-
-        // ---- pass 1: radix R1 on sub-transforms of length P1
-        {
-            float2 x[PER1][R1];
+                for (int r = 0; r < R0; r++) {
+                    // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
+                    // fine rotator first, then one contracted complex product
+                    const float2 lv = lut[bp[r * (T0 / 2)]];
+                    const float l = odd ? lv.y : lv.x;
+                    x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (T0 / SUB)]);
+                }
+                dft<R0>(x);
+                float2* wr = buf + pad(i * R0);
 #pragma unroll
-            for (int m = 0; m < PER1; m++) {
-                const int u = tid + m * NT;
-                const int pol = u / T1, i = u % T1;
-                const int k = i & (P1 - 1);
-#pragma unroll
-                for (int r = 0; r < R1; r++) x[m][r] = buf[pol * FP + pad(i + r * T1)];
-#pragma unroll
-                for (int r = 1; r < R1; r++) x[m][r] = cmul(x[m][r], tw1[r * k]);
-                dft<R1>(x[m]);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int m = 0; m < PER1; m++) {
-                const int u = tid + m * NT;
-                const int pol = u / T1, i = u % T1;
-                const int k = i & (P1 - 1);
-                const int j = (i - k) * R1 + k;
-#pragma unroll
-                for (int r = 0; r < R1; r++) buf[pol * FP + pad(j + r * P1)] = x[m][r];
+                for (int r = 0; r < R0; r++) wr[r] = x[r];
             }
         }
         __syncthreads();
 
-        // ---- pass 2 (last): radix R2, lower half only, frac-sample rotation
+        // ---- middle passes
+        mid_pass<LG, R1, R0, true>(buf, tw1, args.tw, tid);
+        if constexpr (Gm::NP == 4) mid_pass<LG, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
+
+        // ---- last pass: radix RL, channels < nchan only, frac-sample rotation
 #pragma unroll
-        for (int m = 0; m < PER2; m++) {
-            const int u = tid + m * NT;
-            const int pol = u / T2, i = u % T2;
-            float2 x[R2];
+        for (int m = 0; m < PERL; m++) {
+            const int i = tid + m * NT;
+            float2 x[RL];
+            const float2* rd = buf + pad(i);
 #pragma unroll
-            for (int r = 0; r < R2; r++) x[r] = buf[pol * FP + pad(i + r * T2)];
-            twiddle_pow<R2>(x, args.tw, i);
-            dft<R2, true>(x);
+            for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
+            twiddle_pow<RL>(x, args.tw, i);
+            dft<RL, true>(x);
 #pragma unroll
-            for (int r = 0; r < R2 / 2; r++) {
-                const int c = i + r * P2;
-                const float2 fr = cmul_exact(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
-                const float2 y = cmul_exact(fr, x[r]);
+            for (int r = 0; r < RL / 2; r++) {
+                const int c = i + r * PL;
+                const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
+                const float2 y = cmul(fr, x[r]);
                 float si = 1.0f;
-                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + 2 * ant + pol];
-                outr.put_last(m * (R2 / 2) + r, y, si);
+                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + sig];
+                outr.put_last(m * (RL / 2) + r, y, si);
                 if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
             }
         }
     }
 
     // ---- stores: corner-turned [chan][group][Spad][G]
+    tid = threadIdx.x;
 #pragma unroll
-    for (int m = 0; m < PER2; m++) {
-        const int u = tid + m * NT;
-        const int pol = u / T2, i = u % T2;
-        const int sig = 2 * ant + pol;
+    for (int m = 0; m < PERL; m++) {
+        const int i = tid + m * NT;
 #pragma unroll
-        for (int r = 0; r < R2 / 2; r++) {
-            const int c = i + r * P2;
-            const int ou = m * (R2 / 2) + r;
+        for (int r = 0; r < RL / 2; r++) {
+            const int c = i + r * PL;
+            const int ou = m * (RL / 2) + r;
             const long long row = ((long long)c * args.nfg + fgo) * p.Spad + sig;
             if constexpr (PREC == FX_PREC_F16) {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
@@ -550,58 +598,42 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* _
                                                                  float2* __restrict__ out,
                                                                  const float2* __restrict__ tw)
 {
+    // one block per transform, with the F-stage's passes
     using Gm = Geo<LG>;
-    constexpr int F = Gm::F, NT = Gm::NT, R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2;
-    constexpr int T0 = F / R0, T1 = F / R1, T2 = F / R2, P1 = R0, P2 = R0 * R1;
-    constexpr int PER1 = (T1 + NT - 1) / NT, PER2 = (T2 + NT - 1) / NT;
-    __shared__ __align__(16) float2 buf[F + F / 16];
+    constexpr int F = Gm::F, NT = Gm::NT, FP = Gm::FP;
+    constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2, RL = Gm::RL, PL = Gm::PL;
+    constexpr int T0 = F / R0, PER0 = T0 / NT, TL = F / RL, PERL = TL / NT;
+    __shared__ __align__(16) float2 buf[FP];
+    __shared__ float2 tw1[R0 * R1];
     const float2* x_in = in + (long long)blockIdx.x * F;
     float2* x_out = out + (long long)blockIdx.x * F;
     const int tid = threadIdx.x;
-    {
+    for (int e = tid; e < R0 * R1; e += NT) tw1[e] = tw[e * (F / (R0 * R1))];
+#pragma unroll
+    for (int m = 0; m < PER0; m++) {
+        const int i = tid + m * NT;
         float2 x[R0];
 #pragma unroll
-        for (int r = 0; r < R0; r++) x[r] = x_in[tid + r * T0];
+        for (int r = 0; r < R0; r++) x[r] = x_in[i + r * T0];
         dft<R0>(x);
 #pragma unroll
-        for (int r = 0; r < R0; r++) buf[pad(tid * R0 + r)] = x[r];
+        for (int r = 0; r < R0; r++) buf[pad(i * R0 + r)] = x[r];
     }
     __syncthreads();
-    {
-        float2 x[PER1][R1];
+    mid_pass<LG, R1, R0, true>(buf, tw1, tw, tid);
+    if constexpr (Gm::NP == 4) mid_pass<LG, R2, R0 * R1, false>(buf, tw1, tw, tid);
 #pragma unroll
-        for (int m = 0; m < PER1; m++) {
-            const int i = tid + m * NT, k = i & (P1 - 1);
-            if (i >= T1) continue;
-#pragma unroll
-            for (int r = 0; r < R1; r++) x[m][r] = buf[pad(i + r * T1)];
-#pragma unroll
-            for (int r = 1; r < R1; r++) x[m][r] = cmul(x[m][r], tw[(r * k) * (F / (P1 * R1))]);
-            dft<R1>(x[m]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < PER1; m++) {
-            const int i = tid + m * NT, k = i & (P1 - 1), j = (i - k) * R1 + k;
-            if (i >= T1) continue;
-#pragma unroll
-            for (int r = 0; r < R1; r++) buf[pad(j + r * P1)] = x[m][r];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < PER2; m++) {
+    for (int m = 0; m < PERL; m++) {
         const int i = tid + m * NT;
-        if (i >= T2) continue;
-        float2 x[R2];
+        float2 x[RL];
 #pragma unroll
-        for (int r = 0; r < R2; r++) x[r] = buf[pad(i + r * T2)];
+        for (int r = 0; r < RL; r++) x[r] = buf[pad(i + r * TL)];
+        twiddle_pow<RL>(x, tw, i);
+        dft<RL>(x);
 #pragma unroll
-        for (int r = 1; r < R2; r++) x[r] = cmul(x[r], tw[r * i]);
-        dft<R2>(x);
-#pragma unroll
-        for (int r = 0; r < R2; r++) x_out[i + r * P2] = x[r];
+        for (int r = 0; r < RL; r++) x_out[i + r * PL] = x[r];
     }
+    (void)FP;
 }
 
 template <int LG>
@@ -629,7 +661,7 @@ bool fstage_supported(int lg) { return lg >= 9 && lg <= 13; }
 template <int LG>
 static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
 {
-    const int nblocks = a.ngroups * a.p.nant;
+    const int nblocks = 2 * a.ngroups * a.p.nant;     // one block per polarisation
     if (nblocks <= 0) return hipSuccess;
     const dim3 grid(nblocks), block(Geo<LG>::NT);
     const bool tap = a.tap != nullptr;

@@ -28,9 +28,6 @@ struct XArgs {
     int nfg;                      // groups held by the buffer (row stride)
     int fg0, ngroups;             // group range reduced by this launch
     float2* vacc;                 // [nchan][nbp][32*32]
-    const int4* tasks;            // (I, J0, nblocks, bp0) per wave task
-    int ntasks;
-    int ngroups_wg;               // wave tasks per workgroup
     int accumulate;               // 0: overwrite, 1: add
     const float* scale;           // FP8: [nchan][Spad] scales
 };
@@ -44,6 +41,8 @@ bool fstage_supported(int lg);
 // Batched forward C2C FFT of length 2^lg (natural order in/out, unnormalised).
 hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int lg, int batch, hipStream_t s);
 
+// X-engine; ngroups must be a multiple of xengine_chunk_groups(prec).
+int xengine_chunk_groups(int prec);
 hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s);
 hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s);
 hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, hipStream_t s);

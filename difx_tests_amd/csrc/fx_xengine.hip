@@ -14,10 +14,11 @@
 // lane -> k assignment, so the sum is over every k whatever order the
 // instruction reduces them in.
 //
-// A wave owns one row block I and up to NBW column blocks J0..J0+NBW-1, with
-// the accumulators resident for the whole FFT range of the launch; every lane
-// loads its K slice directly with one 16-byte load per block (the F-stage
-// wrote the corner-turned [chan][group][signal][G] layout for this).
+// The F-stage wrote the corner-turned [chan][group][signal][G] layout, so a
+// 16-byte row holds exactly one lane's K slice of one signal; K chunks of
+// two signal panels are staged through LDS and each wave keeps the
+// accumulators of up to NBW blocks resident for the whole FFT range of the
+// launch.
 #include <hip/hip_runtime.h>
 
 #include "fx_common.h"
@@ -29,8 +30,8 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
-constexpr int NBW = 4;          // column blocks per wave task
-constexpr int WAVES = 4;        // wave tasks per workgroup
+constexpr int NBW = 4;          // 32x32 blocks per wave
+constexpr int WAVES = 4;        // waves per workgroup
 
 // (re, im) fp16 pairs -> (im, -re)
 __device__ __forceinline__ uint4 conj_swap_f16(uint4 x)
@@ -50,6 +51,13 @@ __device__ __forceinline__ half8 as_half8(uint4 v)
     half8 h;
     __builtin_memcpy(&h, &v, sizeof(h));
     return h;
+}
+
+__device__ __forceinline__ float4 as_float4(uint4 v)
+{
+    float4 f;
+    __builtin_memcpy(&f, &v, sizeof(f));
+    return f;
 }
 
 __device__ __forceinline__ void store_block(const XArgs& a, int c, int bp, int I, int J, int lane,
@@ -78,169 +86,391 @@ __device__ __forceinline__ void store_block(const XArgs& a, int c, int bp, int I
     }
 }
 
-template <int PREC>
-__global__ __launch_bounds__(64 * WAVES) void fx_xengine_kernel(XArgs a)
+// K-chunk staging: groups of G FFTs per LDS stage, per precision.
+template <int PREC> struct XChunk;
+template <> struct XChunk<FX_PREC_F32> { static constexpr int KC = 4; };   // 8 FFTs
+template <> struct XChunk<FX_PREC_F16> { static constexpr int KC = 4; };   // 16 FFTs
+template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64 FFTs
+
+// One workgroup = one channel x one super-tile pair (SI, SJ) of SB*32-signal
+// panels.  The pair's 32x32 blocks are dealt round-robin to the WAVES waves
+// (at most NBW each).  K chunks of both panels are staged through LDS by all
+// threads with coalesced 16-byte loads, double-buffered: the global loads of
+// chunk k+1 are in flight while the MFMAs of chunk k run.
+template <int PREC, int SB>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2))) void fx_xengine_kernel(XArgs a)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ntg = (a.ntasks + a.ngroups_wg - 1) / a.ngroups_wg;
-    // XCD-aware order: the task groups of one channel share an L2
+    constexpr int KC = XChunk<PREC>::KC;
+    constexpr int PW = SB * FX_BLK;                      // signals per panel
+    constexpr int NELEM = 2 * PW * KC;                   // 16-byte rows per chunk (both panels)
+    constexpr int NLD = (NELEM + 64 * WAVES - 1) / (64 * WAVES);
+    __shared__ __align__(16) uint4 lds[2][2][KC][PW];    // [stage][panel][group][signal]
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
+    const int Spad = a.p.Spad, nblk = a.p.nblk;
+    const int npan = (nblk + SB - 1) / SB;
+    const int npairs = npan * (npan + 1) / 2;
+    // XCD-aware order: the pairs of one channel share an L2
     const int nblocks = gridDim.x;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
     const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
     const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-    const int c = lid / ntg, tg = lid - c * ntg;
-    const int task = tg * a.ngroups_wg + wave;
-    if (task >= a.ntasks || wave >= a.ngroups_wg) return;
-    const int4 t = a.tasks[task];
-    const int I = t.x, J0 = t.y, nb = t.z, bp0 = t.w;
-    const int row = lane & 31, kg = lane >> 5;
-    const int Spad = a.p.Spad;
+    const int c = lid / npairs;
+    int pr = lid - c * npairs, SI = 0;
+    while (pr >= npan - SI) {
+        pr -= npan - SI;
+        SI++;
+    }
+    const int SJ = SI + pr;
+    const bool diag = SI == SJ;
+
+    // blocks of this wave
+    int bI[NBW], bJ[NBW], nb = 0;
+    {
+        const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
+#pragma unroll
+        for (int t = 0; t < NBW; t++) {
+            const int q = wave + t * WAVES;
+            int bi = 0, bj = 0;
+            if (diag) {
+                int r = q;
+                while (bi < SB && r >= SB - bi) {
+                    r -= SB - bi;
+                    bi++;
+                }
+                bj = bi + r;
+            } else {
+                bi = q / SB;
+                bj = q % SB;
+            }
+            const int I = SI * SB + bi, J = SJ * SB + bj;
+            const bool ok = q < nq && I < nblk && J < nblk;
+            bI[t] = ok ? bi : 0;
+            bJ[t] = ok ? bj : 0;
+            if (ok) nb = t + 1;
+        }
+    }
+
+    const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
+    const int nchunk = a.ngroups / KC;
+    // per-thread chunk loads: element e = (panel, group, signal), signal fastest
+    uint4 st[NLD];
+    auto gload = [&](int chunk) {
+#pragma unroll
+        for (int l = 0; l < NLD; l++) {
+            const int e = tid + l * 64 * WAVES;
+            const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+            const int g = rem / PW, sgl = rem - g * PW;
+            const int sig = (pan ? SJ : SI) * PW + sgl;
+            st[l] = make_uint4(0, 0, 0, 0);
+            if (e < NELEM && sig < Spad && !(diag && pan))
+                st[l] = gbase[(long long)(chunk * KC + g) * Spad + sig];
+        }
+    };
+    auto sstore = [&](int stage) {
+#pragma unroll
+        for (int l = 0; l < NLD; l++) {
+            const int e = tid + l * 64 * WAVES;
+            if (e < NELEM) {
+                const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+                const int g = rem / PW, sgl = rem - g * PW;
+                lds[stage][pan][g][sgl] = st[l];
+            }
+        }
+    };
 
     f32x16 accr[NBW], acci[NBW];
 #pragma unroll
-    for (int b = 0; b < NBW; b++) {
-        accr[b] = (f32x16){};
-        acci[b] = (f32x16){};
+    for (int t = 0; t < NBW; t++) {
+        accr[t] = (f32x16){};
+        acci[t] = (f32x16){};
     }
+    const int row = lane & 31, kg = lane >> 5;
+    const int pb = diag ? 0 : 1;                          // panel of the B operand
 
-    if constexpr (PREC == FX_PREC_F16) {
-        // rows of 16 bytes: (re,im) fp16 for G = 4 FFTs
-        const uint4* base = reinterpret_cast<const uint4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
-        const int nks = a.ngroups >> 1;          // K step = 2 groups = 8 FFTs
-        const uint4* pa = base + (long long)kg * Spad + FX_BLK * I + row;
-        const uint4* pb = base + (long long)kg * Spad + FX_BLK * J0 + row;
-        uint4 A = pa[0];
-        uint4 B[NBW];
+    if (nchunk > 0) {
+        gload(0);
+        sstore(0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nchunk; ch++) {
+        const int stg = ch & 1;
+        if (ch + 1 < nchunk) gload(ch + 1);
+        if constexpr (PREC == FX_PREC_F16) {
 #pragma unroll
-        for (int b = 0; b < NBW; b++) B[b] = (b < nb) ? pb[FX_BLK * b] : make_uint4(0, 0, 0, 0);
-        for (int ks = 0; ks < nks; ks++) {
-            uint4 An = A, Bn[NBW];
+            for (int ks = 0; ks < KC / 2; ks++) {
 #pragma unroll
-            for (int b = 0; b < NBW; b++) Bn[b] = B[b];
-            if (ks + 1 < nks) {
-                const long long off = (long long)(ks + 1) * 2 * Spad;
-                An = pa[off];
-#pragma unroll
-                for (int b = 0; b < NBW; b++)
-                    if (b < nb) Bn[b] = pb[off + FX_BLK * b];
-            }
-            const half8 ha = as_half8(A);
-            const half8 hc = as_half8(conj_swap_f16(A));
-#pragma unroll
-            for (int b = 0; b < NBW; b++) {
-                if (b < nb) {
-                    const half8 hb = as_half8(B[b]);
-                    accr[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ha, hb, accr[b], 0, 0, 0);
-                    acci[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hc, hb, acci[b], 0, 0, 0);
-                }
-            }
-            A = An;
-#pragma unroll
-            for (int b = 0; b < NBW; b++) B[b] = Bn[b];
-        }
-    } else if constexpr (PREC == FX_PREC_F32) {
-        // rows of 16 bytes: (re0, re1, im0, im1) for G = 2 FFTs; K step = 1 FFT
-        // (lanes 0-31 supply re, lanes 32-63 im)
-        const float4* base = reinterpret_cast<const float4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
-        const float4* pa = base + FX_BLK * I + row;
-        const float4* pb = base + FX_BLK * J0 + row;
-        float4 A = pa[0];
-        float4 B[NBW];
-#pragma unroll
-        for (int b = 0; b < NBW; b++) B[b] = (b < nb) ? pb[FX_BLK * b] : make_float4(0, 0, 0, 0);
-        for (int g = 0; g < a.ngroups; g++) {
-            float4 An = A, Bn[NBW];
-#pragma unroll
-            for (int b = 0; b < NBW; b++) Bn[b] = B[b];
-            if (g + 1 < a.ngroups) {
-                const long long off = (long long)(g + 1) * Spad;
-                An = pa[off];
-#pragma unroll
-                for (int b = 0; b < NBW; b++)
-                    if (b < nb) Bn[b] = pb[off + FX_BLK * b];
-            }
-            const float av[2] = {kg ? A.z : A.x, kg ? A.w : A.y};
-            const float cv[2] = {kg ? -A.x : A.z, kg ? -A.y : A.w};
-#pragma unroll
-            for (int b = 0; b < NBW; b++) {
-                if (b < nb) {
-                    const float bv[2] = {kg ? B[b].z : B[b].x, kg ? B[b].w : B[b].y};
-#pragma unroll
-                    for (int f = 0; f < 2; f++) {
-                        accr[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[f], bv[f], accr[b], 0, 0, 0);
-                        acci[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[f], bv[f], acci[b], 0, 0, 0);
+                for (int t = 0; t < NBW; t++) {
+                    if (t < nb) {
+                        const uint4 A = lds[stg][0][2 * ks + kg][bI[t] * FX_BLK + row];
+                        const uint4 B = lds[stg][pb][2 * ks + kg][bJ[t] * FX_BLK + row];
+                        const half8 hb = as_half8(B);
+                        accr[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(A), hb, accr[t], 0, 0, 0);
+                        acci[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(conj_swap_f16(A)), hb, acci[t], 0,
+                                                                         0, 0);
                     }
                 }
             }
-            A = An;
+        } else if constexpr (PREC == FX_PREC_F32) {
+            // rows (re0, re1, im0, im1); K step = 1 FFT, lanes 0-31 re, 32-63 im
 #pragma unroll
-            for (int b = 0; b < NBW; b++) B[b] = Bn[b];
-        }
-    } else {
-        // FP8 e4m3: rows of 16 bytes = 8 FFTs; K step = 4 groups = 32 FFTs,
-        // each lane supplies two consecutive groups (32 bytes).
-        const uint4* base = reinterpret_cast<const uint4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
-        const int nks = a.ngroups >> 2;
-        const uint4* pa = base + (long long)(2 * kg) * Spad + FX_BLK * I + row;
-        const uint4* pb = base + (long long)(2 * kg) * Spad + FX_BLK * J0 + row;
-        for (int ks = 0; ks < nks; ks++) {
-            const long long off = (long long)ks * 4 * Spad;
-            const uint4 a0 = pa[off], a1 = pa[off + Spad];
-            i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-            i32x8 cv;
+            for (int g = 0; g < KC; g++) {
 #pragma unroll
-            for (int q = 0; q < 8; q++) cv[q] = (int)conj_swap_f8w((uint32_t)av[q]);
+                for (int t = 0; t < NBW; t++) {
+                    if (t < nb) {
+                        const float4 A = as_float4(lds[stg][0][g][bI[t] * FX_BLK + row]);
+                        const float4 B = as_float4(lds[stg][pb][g][bJ[t] * FX_BLK + row]);
+                        const float av[2] = {kg ? A.z : A.x, kg ? A.w : A.y};
+                        const float cv[2] = {kg ? -A.x : A.z, kg ? -A.y : A.w};
+                        const float bv[2] = {kg ? B.z : B.x, kg ? B.w : B.y};
 #pragma unroll
-            for (int b = 0; b < NBW; b++) {
-                if (b < nb) {
-                    const uint4 b0 = pb[off + FX_BLK * b], b1 = pb[off + Spad + FX_BLK * b];
-                    const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
-                                      (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-                    accr[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, accr[b], 0, 0, 0, 127, 0, 127);
-                    acci[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, bv, acci[b], 0, 0, 0, 127, 0, 127);
+                        for (int f = 0; f < 2; f++) {
+                            accr[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[f], bv[f], accr[t], 0, 0, 0);
+                            acci[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[f], bv[f], acci[t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        } else {
+            // FP8 e4m3: K step = 4 groups (32 FFTs); lane kg supplies groups 2kg, 2kg+1
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ks++) {
+#pragma unroll
+                for (int t = 0; t < NBW; t++) {
+                    if (t < nb) {
+                        const uint4 a0 = lds[stg][0][4 * ks + 2 * kg][bI[t] * FX_BLK + row];
+                        const uint4 a1 = lds[stg][0][4 * ks + 2 * kg + 1][bI[t] * FX_BLK + row];
+                        const uint4 b0 = lds[stg][pb][4 * ks + 2 * kg][bJ[t] * FX_BLK + row];
+                        const uint4 b1 = lds[stg][pb][4 * ks + 2 * kg + 1][bJ[t] * FX_BLK + row];
+                        const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                                          (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                        i32x8 cv;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) cv[q] = (int)conj_swap_f8w((uint32_t)av[q]);
+                        const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
+                                          (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+                        accr[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, accr[t], 0, 0, 0, 127, 0, 127);
+                        acci[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, bv, acci[t], 0, 0, 0, 127, 0, 127);
+                    }
                 }
             }
         }
+        if (ch + 1 < nchunk) sstore(stg ^ 1);
+        __syncthreads();
     }
 
     const float* sc = (PREC == FX_PREC_F8) ? a.scale + (long long)c * Spad : nullptr;
 #pragma unroll
-    for (int b = 0; b < NBW; b++)
-        if (b < nb) store_block(a, c, bp0 + b, I, J0 + b, lane, accr[b], acci[b], sc);
+    for (int t = 0; t < NBW; t++) {
+        if (t < nb) {
+            const int I = SI * SB + bI[t], J = SJ * SB + bJ[t];
+            store_block(a, c, fx_bp_index(I, J, nblk), I, J, lane, accr[t], acci[t], sc);
+        }
+    }
 }
 
-hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s)
+// Plain FP32 X-engine on the vector ALUs (no MFMA): the same workgroup
+// decomposition, LDS staging and accumulator layout as the MFMA kernels, with
+// each lane accumulating one column n and 16 rows m of a 32x32 block by
+// scalar complex multiply-adds (the structure of the reference's CCAH3,
+// gkernel.cu:298-365, register-tiled).
+template <int SB>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2))) void fx_xengine_vec_kernel(XArgs a)
 {
-    const int ntg = (a.ntasks + a.ngroups_wg - 1) / a.ngroups_wg;
-    const int nblocks = a.p.nchan * ntg;
-    if (nblocks <= 0 || a.ngroups <= 0) return hipSuccess;
-    const dim3 grid(nblocks), block(64 * a.ngroups_wg);
-    switch (prec) {
-    case FX_PREC_F32: fx_xengine_kernel<FX_PREC_F32><<<grid, block, 0, s>>>(a); break;
-    case FX_PREC_F16: fx_xengine_kernel<FX_PREC_F16><<<grid, block, 0, s>>>(a); break;
-    case FX_PREC_F8:  fx_xengine_kernel<FX_PREC_F8><<<grid, block, 0, s>>>(a); break;
-    default: return hipErrorInvalidValue;
+    constexpr int KC = XChunk<FX_PREC_F32>::KC;
+    constexpr int PW = SB * FX_BLK;
+    constexpr int NELEM = 2 * PW * KC;
+    constexpr int NLD = (NELEM + 64 * WAVES - 1) / (64 * WAVES);
+    constexpr int NBV = 3;                                // blocks per wave (10 blocks / 4 waves)
+    __shared__ __align__(16) float4 lds[2][2][KC][PW];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
+    const int Spad = a.p.Spad, nblk = a.p.nblk;
+    const int npan = (nblk + SB - 1) / SB;
+    const int npairs = npan * (npan + 1) / 2;
+    const int nblocks = gridDim.x;
+    const int q8 = nblocks >> 3, r8 = nblocks & 7;
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
+    const int c = lid / npairs;
+    int pr = lid - c * npairs, SI = 0;
+    while (pr >= npan - SI) {
+        pr -= npan - SI;
+        SI++;
+    }
+    const int SJ = SI + pr;
+    const bool diag = SI == SJ;
+    const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
+    const int pb = diag ? 0 : 1;
+    const float4* gbase = reinterpret_cast<const float4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
+    const int nchunk = a.ngroups / KC;
+    const int n = lane & 31, kg = lane >> 5;
+
+    for (int q0 = 0; q0 < nq; q0 += WAVES * NBV) {
+        // this wave's blocks
+        int bI[NBV], bJ[NBV], nb = 0;
+#pragma unroll
+        for (int t = 0; t < NBV; t++) {
+            const int q = q0 + wave + t * WAVES;
+            int bi = 0, bj = 0;
+            if (diag) {
+                int r = q;
+                while (bi < SB && r >= SB - bi) {
+                    r -= SB - bi;
+                    bi++;
+                }
+                bj = bi + r;
+            } else {
+                bi = q / SB;
+                bj = q % SB;
+            }
+            const bool ok = q < nq && SI * SB + bi < nblk && SJ * SB + bj < nblk;
+            bI[t] = ok ? bi : 0;
+            bJ[t] = ok ? bj : 0;
+            if (ok) nb = t + 1;
+        }
+        float accr[NBV][16], acci[NBV][16];
+#pragma unroll
+        for (int t = 0; t < NBV; t++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) accr[t][v] = acci[t][v] = 0.0f;
+        float4 st[NLD];
+        auto gload = [&](int chunk) {
+#pragma unroll
+            for (int l = 0; l < NLD; l++) {
+                const int e = tid + l * 64 * WAVES;
+                const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+                const int g = rem / PW, sgl = rem - g * PW;
+                const int sig = (pan ? SJ : SI) * PW + sgl;
+                st[l] = make_float4(0, 0, 0, 0);
+                if (e < NELEM && sig < Spad && !(diag && pan)) st[l] = gbase[(long long)(chunk * KC + g) * Spad + sig];
+            }
+        };
+        auto sstore = [&](int stage) {
+#pragma unroll
+            for (int l = 0; l < NLD; l++) {
+                const int e = tid + l * 64 * WAVES;
+                if (e < NELEM) {
+                    const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+                    const int g = rem / PW, sgl = rem - g * PW;
+                    lds[stage][pan][g][sgl] = st[l];
+                }
+            }
+        };
+        __syncthreads();
+        if (nchunk > 0) {
+            gload(0);
+            sstore(0);
+        }
+        __syncthreads();
+        for (int ch = 0; ch < nchunk; ch++) {
+            const int stg = ch & 1;
+            if (ch + 1 < nchunk) gload(ch + 1);
+#pragma unroll
+            for (int g = 0; g < KC; g++) {
+#pragma unroll
+                for (int t = 0; t < NBV; t++) {
+                    if (t < nb) {
+                        const float4 B = lds[stg][pb][g][bJ[t] * FX_BLK + n];
+#pragma unroll
+                        for (int v = 0; v < 16; v++) {
+                            const int m = (v >> 2) * 8 + kg * 4 + (v & 3);
+                            const float4 A = lds[stg][0][g][bI[t] * FX_BLK + m];
+                            // A * conj(B) for the two FFTs of the row
+                            accr[t][v] += A.x * B.x + A.z * B.z;
+                            acci[t][v] += A.z * B.x - A.x * B.z;
+                            accr[t][v] += A.y * B.y + A.w * B.w;
+                            acci[t][v] += A.w * B.y - A.y * B.w;
+                        }
+                    }
+                }
+            }
+            if (ch + 1 < nchunk) sstore(stg ^ 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int t = 0; t < NBV; t++) {
+            if (t < nb) {
+                f32x16 re, im;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    re[v] = accr[t][v];
+                    im[v] = acci[t][v];
+                }
+                const int I = SI * SB + bI[t], J = SJ * SB + bJ[t];
+                store_block(a, c, fx_bp_index(I, J, nblk), I, J, lane, re, im, nullptr);
+            }
+        }
+    }
+}
+
+int xengine_super_blocks(int nblk) { return nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1; }
+
+template <int PREC>
+static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
+{
+    const int SB = xengine_super_blocks(a.p.nblk);
+    const int npan = (a.p.nblk + SB - 1) / SB;
+    const int npairs = npan * (npan + 1) / 2;
+    const dim3 grid(a.p.nchan * npairs), block(64 * WAVES);
+    switch (SB) {
+    case 4: fx_xengine_kernel<PREC, 4><<<grid, block, 0, s>>>(a); break;
+    case 2: fx_xengine_kernel<PREC, 2><<<grid, block, 0, s>>>(a); break;
+    default: fx_xengine_kernel<PREC, 1><<<grid, block, 0, s>>>(a); break;
     }
     return hipGetLastError();
 }
 
+int xengine_chunk_groups(int prec) { return prec == FX_PREC_F8 ? XChunk<FX_PREC_F8>::KC : XChunk<FX_PREC_F16>::KC; }
+
+
+hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s)
+{
+    if (a.p.nchan <= 0 || a.ngroups <= 0) return hipSuccess;
+    if (a.ngroups % xengine_chunk_groups(prec)) return hipErrorInvalidValue;   // callers pad to whole chunks
+    switch (prec) {
+    case FX_PREC_F32: return launch_x_prec<FX_PREC_F32>(a, s);
+    case FX_PREC_F16: return launch_x_prec<FX_PREC_F16>(a, s);
+    case FX_PREC_F8:  return launch_x_prec<FX_PREC_F8>(a, s);
+    case FX_PREC_F32V: {
+        const int SB = xengine_super_blocks(a.p.nblk);
+        const int npan = (a.p.nblk + SB - 1) / SB;
+        const dim3 grid(a.p.nchan * (npan * (npan + 1) / 2)), block(64 * WAVES);
+        switch (SB) {
+        case 4: fx_xengine_vec_kernel<4><<<grid, block, 0, s>>>(a); break;
+        case 2: fx_xengine_vec_kernel<2><<<grid, block, 0, s>>>(a); break;
+        default: fx_xengine_vec_kernel<1><<<grid, block, 0, s>>>(a); break;
+        }
+        return hipGetLastError();
+    }
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // counts[j*nant + k] (j <= k) += number of FFTs where both antennas are valid
 // (kernel.c:850; j == k counts autocorrelation FFTs).
+// One thread per (antenna pair j <= k, chunk of 32 FFTs); the per-pair sums
+// of the chunks meet in counts[j*nant + k] through atomic adds.
 __global__ void fx_counts_kernel(int nant, int nffts, const FxDelay* __restrict__ d, int* __restrict__ counts)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nant * nant) return;
-    const int j = t / nant, k = t - j * nant;
-    if (j > k) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int pairs = nant * nant;
+    const int chunk = (int)(t / pairs), jk = (int)(t - (long long)chunk * pairs);
+    const int j = jk / nant, k = jk - j * nant;
+    const int f0 = chunk * 32;
+    if (f0 >= nffts || j > k) return;
+    const int f1 = f0 + 32 < nffts ? f0 + 32 : nffts;
     int n = 0;
-    for (int f = 0; f < nffts; f++) n += d[(long long)f * nant + j].valid & d[(long long)f * nant + k].valid;
-    counts[t] += n;
+#pragma unroll 8
+    for (int f = f0; f < f1; f++) n += d[(long long)f * nant + j].valid & d[(long long)f * nant + k].valid;
+    if (n) atomicAdd(counts + jk, n);
 }
 
 hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s)
 {
-    const int n = nant * nant;
-    fx_counts_kernel<<<(n + 255) / 256, 256, 0, s>>>(nant, nffts, delays, counts);
+    const long long n = (long long)nant * nant * ((nffts + 31) / 32);
+    if (n <= 0) return hipSuccess;
+    fx_counts_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nant, nffts, delays, counts);
     return hipGetLastError();
 }
 

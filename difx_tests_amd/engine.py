@@ -14,8 +14,8 @@ import numpy as np
 LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 LIBPATH = os.path.join(LIBDIR, "libdifxfx.so")
 
-PREC_F32, PREC_F16, PREC_F8 = 0, 1, 2
-PREC_NAMES = {"f32": PREC_F32, "f16": PREC_F16, "f8": PREC_F8}
+PREC_F32, PREC_F16, PREC_F8, PREC_F32V = 0, 1, 2, 3
+PREC_NAMES = {"f32": PREC_F32, "f16": PREC_F16, "f8": PREC_F8, "f32v": PREC_F32V}
 
 # Every symbol include/difxfx.h declares.
 NATIVE_SYMBOLS = [

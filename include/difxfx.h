@@ -31,6 +31,7 @@ extern "C" {
 #define FX_PREC_F32 0   /* spectra and X-engine in FP32 (plain path)            */
 #define FX_PREC_F16 1   /* spectra cast to FP16, MFMA with FP32 accumulation     */
 #define FX_PREC_F8  2   /* spectra scaled per (channel, signal) to FP8 e4m3      */
+#define FX_PREC_F32V 3  /* FP32 spectra, plain FP32 vector-ALU X-engine (no MFMA) */
 
 typedef struct fx_context fx_context;
 

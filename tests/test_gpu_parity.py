@@ -96,10 +96,13 @@ def test_fstage_fp32_matches_oracle(nchan):
     assert not np.any(chan[~v])                          # invalid slots are zero
 
 
-@pytest.mark.parametrize("nchan,nant,nf", [(256, 4, 16), (1024, 4, 9), (2048, 6, 5), (4096, 3, 4), (512, 17, 6)])
-def test_vis_fp32_matches_oracle(nchan, nant, nf):
+@pytest.mark.parametrize("prec", ["f32", "f32v"])
+@pytest.mark.parametrize("nchan,nant,nf", [(256, 4, 16), (1024, 4, 9), (2048, 6, 5), (4096, 3, 4), (512, 17, 6),
+                                           (256, 40, 8)])
+def test_vis_fp32_matches_oracle(nchan, nant, nf, prec):
+    # f32: FP32 MFMA X-engine; f32v: plain FP32 vector X-engine
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 30 + nchan + nant)
-    with _engine(p, nant, nf, data=data) as e:
+    with _engine(p, nant, nf, prec=prec, data=data) as e:
         e.set_model(delays, fileoff)
         e.upload(data)
         e.reset()
