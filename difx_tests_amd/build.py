@@ -84,6 +84,23 @@ def build_oracle() -> str:
     return so
 
 
+def build_test_drivers() -> str:
+    """C driver of the per-(antenna, FFT) kernel.h functions (tests only)."""
+    outdir = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    exe = os.path.join(outdir, "kernelh_driver")
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", f"-I{INCLUDE}", os.path.join(ROOT, "tests", "kernelh_driver.c"),
+          "-o", exe, f"-L{LIBDIR}", "-ldifxfx", "-Wl,-rpath,$ORIGIN/../../difx_tests_amd/lib",
+          "-Wl,-rpath,/opt/rocm/lib", "-lm"])
+    return exe
+
+
+def build_reference_host() -> None:
+    """The reference's main.c linked against this library, when the reference is present."""
+    if os.path.exists("/root/reference/main.c"):
+        _run(["bash", os.path.join(ORACLE, "build_ref_host.sh")])
+
+
 def build_all(oracle: bool = True) -> None:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
@@ -91,6 +108,8 @@ def build_all(oracle: bool = True) -> None:
     build_host()
     if oracle:
         build_oracle()
+        build_test_drivers()
+        build_reference_host()
 
 
 if __name__ == "__main__":
