@@ -87,6 +87,20 @@ def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch):
         e.close()
 
 
+def measured_traffic(workload, prec, xbatch, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{prec}_xbatch{xbatch}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(w, data, delays, fileoff, budget_s):
     """Single-thread FP32 restatement (oracle) on a bounded FFT sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -139,13 +153,18 @@ def main():
     f_gbs = f_launch_bytes / (f_ms * 1e-3) / 1e9
     x_tflops = x_flops / (x_ms * 1e-3) / 1e12
     if kms["fstage"] >= kms["xstage"]:
+        traffic, src = measured_traffic(w.name, prec, a.xbatch, "fx_fstage_kernel")
         roof = {"kernel": "fx_fstage_kernel", "bound": "hbm", "achieved": round(f_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(f_ms, 4)}
+                "unit": "GB/s", "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": int(f_launch_bytes), "avg_launch_ms": round(f_ms, 4)}
     else:
+        traffic, src = measured_traffic(w.name, prec, a.xbatch, "fx_xengine_kernel")
         pk = MFMA_PEAK_TFLOPS[prec]
         roof = {"kernel": "fx_xengine_kernel", "bound": "mfma", "achieved": round(x_tflops, 2), "peak": pk,
-                "unit": "TFLOP/s", "frac": round(x_tflops / pk, 4), "traffic": None, "avg_launch_ms": round(x_ms, 4)}
+                "unit": "TFLOP/s", "frac": round(x_tflops / pk, 4), "traffic": traffic,
+                "algorithmic_flops": int(x_flops), "avg_launch_ms": round(x_ms, 4)}
+    if src:
+        roof["traffic_source"] = src
 
     out = {
         "metric": "input Gsamples/s/GPU + X-engine TFLOPS; FP16-MFMA vs FP32 at <=1e-3 rel err",

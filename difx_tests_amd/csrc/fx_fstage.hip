@@ -235,6 +235,7 @@ template <> struct FPlan<10> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2
 template <> struct FPlan<11> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1; };
 template <> struct FPlan<12> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
 template <> struct FPlan<13> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2; };
+template <> struct FPlan<14> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4; };
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
@@ -335,7 +336,9 @@ int fx_fstage_group_size(int prec)
 // Middle pass: radix R on sub-transforms of length P (Stockham, in place in
 // LDS with a barrier between the reads and the writes).
 
-template <int LG, int R, int P, bool TW_LDS>
+// TWS is the stride of the twiddle table: 1 when tw holds exp(-2 pi i k / 2^LG),
+// 2 when a 2^LG-point sub-transform reads the table of a 2^(LG+1)-point one.
+template <int LG, int R, int P, bool TW_LDS, int TWS = 1>
 __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const float2* __restrict__ tw, int tid)
 {
     using Gm = Geo<LG>;
@@ -357,7 +360,7 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
 #pragma unroll
             for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tw1[r * k]);
         } else {
-            twiddle_pow<R>(x[m], tw, k * (F / (P * R)));
+            twiddle_pow<R>(x[m], tw, k * (TWS * F / (P * R)));
         }
         dft<R>(x[m]);
     }
@@ -400,7 +403,7 @@ void fx_fstage_kernel(FStageArgs args)
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    __shared__ __align__(16) float2 lut[256];           // byte -> (level t0, level t1) of this pol
+    __shared__ __align__(16) float lut[2][256];         // [time slot][byte] -> level of this pol
     __shared__ __align__(16) uint32_t bytes[F / 8 + 4];
     __shared__ float2 rsub[SUB], rstep[NSTEP], frsub[STRIDE], frstep[NFSTEP];
     __shared__ float2 tw1[NTW1];                         // exp(-2 pi i m / (R0 R1))
@@ -426,7 +429,8 @@ void fx_fstage_kernel(FStageArgs args)
         // pol p of time slot t sits at bits 2*(2t+p) (kernel.c:195-200)
         const float hi = 3.3359f;
         const float lv[4] = {-hi, -1.0f, 1.0f, hi};
-        lut[e] = make_float2(lv[(e >> (2 * pol)) & 3], lv[(e >> (4 + 2 * pol)) & 3]);
+        lut[0][e] = lv[(e >> (2 * pol)) & 3];
+        lut[1][e] = lv[(e >> (4 + 2 * pol)) & 3];
     }
 
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
@@ -438,6 +442,7 @@ void fx_fstage_kernel(FStageArgs args)
         // from it are then formed where they are used instead of being hoisted
         // out of the loop and held in registers across all passes.
         asm volatile("" : "+v"(tid));
+        __builtin_assume(tid >= 0 && tid < NT);
         outr.shift();
         const int fft = grp * G + gi;             // FFT of this launch
         const FxDelay d = args.delays[(long long)fft * p.nant + ant];
@@ -511,7 +516,7 @@ void fx_fstage_kernel(FStageArgs args)
                 // time slot is fixed per thread and the byte, step-rotator and
                 // LDS write addresses are a base plus r times a constant
                 const int q0 = o + i;
-                const bool odd = q0 & 1;
+                const float* lt = lut[q0 & 1];
                 const uint8_t* bp = bl + ((q0 >> 1) - bbase);
                 const float2* sp = rstep + i / SUB;
                 float2 x[R0];
@@ -519,8 +524,7 @@ void fx_fstage_kernel(FStageArgs args)
                 for (int r = 0; r < R0; r++) {
                     // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
                     // fine rotator first, then one contracted complex product
-                    const float2 lv = lut[bp[r * (T0 / 2)]];
-                    const float l = odd ? lv.y : lv.x;
+                    const float l = lt[bp[r * (T0 / 2)]];
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (T0 / SUB)]);
                 }
                 dft<R0>(x);
@@ -590,6 +594,280 @@ void fx_fstage_kernel(FStageArgs args)
 }
 
 // --------------------------------------------------------------------------
+// F = 32768 (16384 channels).  An FP32 transform of this length does not fit
+// the LDS, so it is formed from its even- and odd-sample halves,
+//   X[c] = E[c] + W^c O[c],   c < F/2 (exactly the channels kept),
+// each half a 16384-point transform on the F = 16384 passes.  E stays in
+// registers while O is formed.  Both halves read the packed bytes straight
+// from global memory (one time slot each, so the byte index is contiguous
+// in the thread index); outputs go to the corner-turned rows one FFT slot at
+// a time.
+
+static constexpr int kEoLG = 15;
+
+template <int PREC>
+__device__ __forceinline__ void store_slot(const FStageArgs& args, long long row, int gi, float2 y, float si)
+{
+    if constexpr (PREC == FX_PREC_F32) {
+        float* dst = reinterpret_cast<float*>(args.out) + row * 4;      // (re0, re1, im0, im1)
+        dst[gi] = y.x;
+        dst[2 + gi] = y.y;
+    } else if constexpr (PREC == FX_PREC_F16) {
+        const __half2 h = __floats2half2_rn(y.x, y.y);
+        reinterpret_cast<uint32_t*>(args.out)[row * 4 + gi] = *reinterpret_cast<const uint32_t*>(&h);
+    } else {
+        const float a = fminf(fmaxf(y.x * si, -448.0f), 448.0f);
+        const float b = fminf(fmaxf(y.y * si, -448.0f), 448.0f);
+        const uint32_t w = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+        reinterpret_cast<uint16_t*>(args.out)[row * 8 + gi] = (uint16_t)(w & 0xffffu);
+    }
+}
+
+template <int PREC, bool TAP>
+__global__ __launch_bounds__(Geo<kEoLG - 1>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+void fx_fstage_eo_kernel(FStageArgs args)
+{
+    using Gh = Geo<kEoLG - 1>;                                  // geometry of one half
+    constexpr int LG = kEoLG, F = 1 << LG, NCH = F / 2, H = F / 2;
+    constexpr int NT = Gh::NT, FP = Gh::FP;
+    constexpr int STRIDE = 1 << ((LG - 1) / 2), SUB = 2 * STRIDE, NSTEP = F / SUB, NFSTEP = NCH / STRIDE;
+    constexpr int NTAB = SUB + NSTEP + STRIDE + NFSTEP;
+    constexpr int R0 = Gh::R0, R1 = Gh::R1, R2 = Gh::R2, RL = Gh::RL;
+    constexpr int T0 = H / R0, TL = H / RL, PERL = TL / NT, PL = H / RL;
+    static_assert(Gh::NP == 4 && T0 == NT, "pass 0: one butterfly per thread");
+    static_assert((2 * T0) % SUB == 0, "pass-0 step rotator is a base plus r times a constant");
+    static_assert(PERL * NT == PL && RL * PL == H, "last pass covers every bin of a half");
+    constexpr int G = GroupSize<PREC>::G;
+    constexpr int NTW1 = R0 * R1;
+
+    __shared__ __align__(16) float2 buf[FP];
+    __shared__ __align__(16) float lut[2][256];
+    __shared__ float2 rsub[SUB], rstep[NSTEP], frsub[STRIDE], frstep[NFSTEP];
+    __shared__ float2 tw1[NTW1];
+
+    const FxPlan& p = args.p;
+    const int nblocks = gridDim.x;
+    const int q8 = nblocks >> 3, r8 = nblocks & 7;
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
+    const int pol = lid & 1;
+    const int grp = (lid >> 1) / p.nant;
+    const int ant = (lid >> 1) - grp * p.nant;
+    const int fgo = args.fg0 + grp;
+    const int sig = 2 * ant + pol;
+    int tid = threadIdx.x;
+
+    for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
+    for (int e = tid; e < 256; e += NT) {
+        const float hi = 3.3359f;
+        const float lv[4] = {-hi, -1.0f, 1.0f, hi};
+        lut[0][e] = lv[(e >> (2 * pol)) & 3];
+        lut[1][e] = lv[(e >> (4 + 2 * pol)) & 3];
+    }
+    const uint8_t* src = args.data + (long long)ant * args.data_stride;
+
+#pragma unroll 1
+    for (int gi = 0; gi < G; gi++) {
+        asm volatile("" : "+v"(tid));
+        __builtin_assume(tid >= 0 && tid < NT);
+        const int fft = grp * G + gi;
+        const FxDelay d = args.delays[(long long)fft * p.nant + ant];
+        __syncthreads();
+        if (!d.valid) {
+#pragma unroll
+            for (int m = 0; m < PERL; m++)
+#pragma unroll
+                for (int r = 0; r < RL; r++) {
+                    const int c = tid + m * NT + r * PL;
+                    store_slot<PREC>(args, ((long long)c * args.nfg + fgo) * p.Spad + sig, gi,
+                                     make_float2(0.0f, 0.0f), 0.0f);
+                    if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = make_float2(0.0f, 0.0f);
+                }
+            continue;
+        }
+        {
+#pragma clang fp contract(off)
+            const int integerdelay = (int)d.b;
+            const double bfrac = d.b - integerdelay;
+            const double lofr = p.lofrac * (double)integerdelay;
+            const float fd32 = (float)d.fd;
+            for (int e = tid; e < NTAB; e += NT) {
+                float arg;
+                bool unit = false;
+                float2* dst;
+                if (e < SUB) {
+                    double subxval = ((double)e / (double)F) * d.a;
+                    subxval += bfrac;
+                    double ph = subxval * p.lo;
+                    if (p.fracLO) ph += lofr;
+                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
+                    dst = rsub + e;
+                } else if (e < SUB + NSTEP) {
+                    const int k = e - SUB;
+                    const double stepxval = (((double)k * (double)STRIDE) / (double)F) * d.a;
+                    const double ph = stepxval * p.lo;
+                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
+                    unit = (k == 0);
+                    dst = rstep + k;
+                } else if (e < SUB + NSTEP + STRIDE) {
+                    const int s = e - SUB - NSTEP;
+                    arg = args.subchf[s] * fd32;
+                    dst = frsub + s;
+                } else {
+                    const int k = e - SUB - NSTEP - STRIDE;
+                    arg = args.stepchf[k] * fd32;
+                    unit = (k == 0);
+                    dst = frstep + k;
+                }
+                *dst = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
+            }
+        }
+        __syncthreads();
+
+        const int o = d.offset;
+        float2 ev[PERL][RL];                                   // E[c], c = tid + m NT + r PL
+#pragma unroll 1
+        for (int h = 0; h < 2; h++) {
+            asm volatile("" : "+v"(tid));
+            __builtin_assume(tid >= 0 && tid < NT);
+            // ---- pass 0 of half h: samples n = 2 (i + r T0) + h, one time slot
+            {
+                const int i = tid;
+                const int n0 = 2 * i + h;
+                const float2 rs = rsub[n0 & (SUB - 1)];
+                const float2* sp = rstep + n0 / SUB;
+                const float* lt = lut[(o + h) & 1];
+                const uint8_t* bp = src + ((o + h) >> 1) + i;
+                float2 x[R0];
+#pragma unroll
+                for (int r = 0; r < R0; r++) {
+                    const float l = lt[bp[r * T0]];
+                    x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (2 * T0 / SUB)]);
+                }
+                dft<R0>(x);
+                float2* wr = buf + pad(i * R0);
+#pragma unroll
+                for (int r = 0; r < R0; r++) wr[r] = x[r];
+            }
+            __syncthreads();
+            mid_pass<LG - 1, R1, R0, true, 2>(buf, tw1, args.tw, tid);
+            mid_pass<LG - 1, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
+            // ---- last pass of half h, every bin
+#pragma unroll
+            for (int m = 0; m < PERL; m++) {
+                const int i = tid + m * NT;
+                float2 x[RL];
+                const float2* rd = buf + pad(i);
+#pragma unroll
+                for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
+                twiddle_pow<RL>(x, args.tw, 2 * i);
+                dft<RL>(x);
+                if (h == 0) {
+#pragma unroll
+                    for (int r = 0; r < RL; r++) ev[m][r] = x[r];
+                } else {
+                    // W^c = W^i * exp(-2 pi i r PL / F); PL / F = 1/8
+                    const float2 wi = args.tw[i];
+#pragma unroll
+                    for (int r = 0; r < RL; r++) {
+                        const int c = i + r * PL;
+                        const float2 ov = twc(cmul(x[r], wi), r * (32 * PL / F));
+                        const float2 X = cadd(ev[m][r], ov);
+                        const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
+                        const float2 y = cmul(fr, X);
+                        float si = 1.0f;
+                        if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + sig];
+                        store_slot<PREC>(args, ((long long)c * args.nfg + fgo) * p.Spad + sig, gi, y, si);
+                        if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
+                    }
+                }
+            }
+            __syncthreads();                                   // buf is rewritten by the next half / FFT
+        }
+    }
+}
+
+template <int PREC>
+static void launch_eo_prec(const FStageArgs& a, dim3 grid, dim3 block, bool tap, hipStream_t s)
+{
+    if (tap) fx_fstage_eo_kernel<PREC, true><<<grid, block, 0, s>>>(a);
+    else fx_fstage_eo_kernel<PREC, false><<<grid, block, 0, s>>>(a);
+}
+
+static hipError_t launch_eo(const FStageArgs& a, int prec, hipStream_t s)
+{
+    const int nblocks = 2 * a.ngroups * a.p.nant;
+    if (nblocks <= 0) return hipSuccess;
+    const dim3 grid(nblocks), block(Geo<kEoLG - 1>::NT);
+    const bool tap = a.tap != nullptr;
+    switch (prec) {
+    case FX_PREC_F32: launch_eo_prec<FX_PREC_F32>(a, grid, block, tap, s); break;
+    case FX_PREC_F16: launch_eo_prec<FX_PREC_F16>(a, grid, block, tap, s); break;
+    case FX_PREC_F8: launch_eo_prec<FX_PREC_F8>(a, grid, block, tap, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Forward C2C FFT of length 32768 (all bins) from the two halves:
+// X[c] = E[c] + W^c O[c], X[c + F/2] = E[c] - W^c O[c].
+__global__ __launch_bounds__(Geo<kEoLG - 1>::NT) void fx_fft_c2c_eo_kernel(const float2* __restrict__ in,
+                                                                         float2* __restrict__ out,
+                                                                         const float2* __restrict__ tw)
+{
+    using Gh = Geo<kEoLG - 1>;
+    constexpr int F = 1 << kEoLG, H = F / 2, NT = Gh::NT, FP = Gh::FP;
+    constexpr int R0 = Gh::R0, R1 = Gh::R1, R2 = Gh::R2, RL = Gh::RL;
+    constexpr int T0 = H / R0, TL = H / RL, PERL = TL / NT, PL = H / RL;
+    __shared__ __align__(16) float2 buf[FP];
+    __shared__ float2 tw1[R0 * R1];
+    const float2* x_in = in + (long long)blockIdx.x * F;
+    float2* x_out = out + (long long)blockIdx.x * F;
+    int tid = threadIdx.x;
+    for (int e = tid; e < R0 * R1; e += NT) tw1[e] = tw[e * (F / (R0 * R1))];
+    float2 ev[PERL][RL];
+#pragma unroll 1
+    for (int h = 0; h < 2; h++) {
+        asm volatile("" : "+v"(tid));
+        __builtin_assume(tid >= 0 && tid < NT);
+        {
+            float2 x[R0];
+#pragma unroll
+            for (int r = 0; r < R0; r++) x[r] = x_in[2 * (tid + r * T0) + h];
+            dft<R0>(x);
+#pragma unroll
+            for (int r = 0; r < R0; r++) buf[pad(tid * R0 + r)] = x[r];
+        }
+        __syncthreads();
+        mid_pass<kEoLG - 1, R1, R0, true, 2>(buf, tw1, tw, tid);
+        mid_pass<kEoLG - 1, R2, R0 * R1, false, 2>(buf, tw1, tw, tid);
+#pragma unroll
+        for (int m = 0; m < PERL; m++) {
+            const int i = tid + m * NT;
+            float2 x[RL];
+#pragma unroll
+            for (int r = 0; r < RL; r++) x[r] = buf[pad(i + r * TL)];
+            twiddle_pow<RL>(x, tw, 2 * i);
+            dft<RL>(x);
+            if (h == 0) {
+#pragma unroll
+                for (int r = 0; r < RL; r++) ev[m][r] = x[r];
+            } else {
+                const float2 wi = tw[i];
+#pragma unroll
+                for (int r = 0; r < RL; r++) {
+                    const float2 ov = twc(cmul(x[r], wi), r * (32 * PL / F));
+                    x_out[i + r * PL] = cadd(ev[m][r], ov);
+                    x_out[i + r * PL + H] = csub(ev[m][r], ov);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    (void)FP;
+}
+
+// --------------------------------------------------------------------------
 // Generic forward C2C FFT (ippsFFTFwd_CToC_32fc semantics: unnormalised, all
 // F bins), one workgroup per transform; same passes as the F-stage.
 
@@ -652,11 +930,15 @@ hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int l
     case 11: return launch_fft_lg<11>(in, out, tw, batch, s);
     case 12: return launch_fft_lg<12>(in, out, tw, batch, s);
     case 13: return launch_fft_lg<13>(in, out, tw, batch, s);
+    case 14: return launch_fft_lg<14>(in, out, tw, batch, s);
+    case 15:
+        fx_fft_c2c_eo_kernel<<<batch, Geo<kEoLG - 1>::NT, 0, s>>>(in, out, tw);
+        return hipGetLastError();
     default: return hipErrorInvalidValue;
     }
 }
 
-bool fstage_supported(int lg) { return lg >= 9 && lg <= 13; }
+bool fstage_supported(int lg) { return lg >= 9 && lg <= 15; }
 
 template <int LG>
 static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
@@ -691,6 +973,8 @@ hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
     case 11: return launch_lg<11>(a, prec, s);
     case 12: return launch_lg<12>(a, prec, s);
     case 13: return launch_lg<13>(a, prec, s);
+    case 14: return launch_lg<14>(a, prec, s);
+    case 15: return launch_eo(a, prec, s);
     default: return hipErrorInvalidValue;
     }
 }

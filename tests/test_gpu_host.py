@@ -95,8 +95,9 @@ def test_reference_main_c_on_this_library(tmp_path):
     _check_visout(str(tmp_path), p, data, delays, fileoff, 32)
 
 
-def test_kernelh_fine_grained_functions(tmp_path):
-    nant, nchan, numffts, g = 3, 512, 6, 4
+@pytest.mark.parametrize("nchan", [512, 16384])
+def test_kernelh_fine_grained_functions(tmp_path, nchan):
+    nant, numffts, g = 3, 6, 4
     p = orc.plan(nchan, LO, BW)
     F = 2 * nchan
     rng = np.random.default_rng(5)

@@ -59,7 +59,7 @@ def _rel(a, b, axis=-1):
     return np.linalg.norm(a - b, axis=axis) / np.maximum(np.linalg.norm(b, axis=axis), 1e-30)
 
 
-@pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096, 8192, 16384])
 def test_delay_table_bit_exact(nchan):
     nant, nf = 5, 12
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 10 + nchan, rate=2e-10, spread=2.2)
@@ -79,7 +79,7 @@ def test_delay_table_bit_exact(nchan):
     assert 0 < ref["valid"].sum() < nf * nant          # the case exercises invalid antennas
 
 
-@pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096, 8192, 16384])
 def test_fstage_fp32_matches_oracle(nchan):
     nant, nf = 3, 6
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 20 + nchan)
@@ -98,7 +98,7 @@ def test_fstage_fp32_matches_oracle(nchan):
 
 @pytest.mark.parametrize("prec", ["f32", "f32v"])
 @pytest.mark.parametrize("nchan,nant,nf", [(256, 4, 16), (1024, 4, 9), (2048, 6, 5), (4096, 3, 4), (512, 17, 6),
-                                           (256, 40, 8)])
+                                           (256, 40, 8), (8192, 3, 3), (16384, 3, 3)])
 def test_vis_fp32_matches_oracle(nchan, nant, nf, prec):
     # f32: FP32 MFMA X-engine; f32v: plain FP32 vector X-engine
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 30 + nchan + nant)
@@ -174,7 +174,7 @@ def test_invalid_antenna_contributes_nothing():
     assert _rel(cross[live], ref["vis"][live]).max() <= 1e-5
 
 
-@pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 32), (4096, 8, 16), (512, 20, 24)])
+@pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 32), (4096, 8, 16), (512, 20, 24), (16384, 4, 8)])
 def test_vis_fp16_vs_fp64_oracle(nchan, nant, nf):
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 40 + nchan, data="correlated")
     with _engine(p, nant, nf, prec="f16", data=data) as e:
@@ -190,7 +190,7 @@ def test_vis_fp16_vs_fp64_oracle(nchan, nant, nf):
     assert _rel(autos, ref["autos"]).max() <= 1e-3
 
 
-@pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 64), (2048, 12, 32)])
+@pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 64), (2048, 12, 32), (16384, 4, 24)])
 def test_vis_fp8_vs_fp64_oracle(nchan, nant, nf):
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 50 + nchan, data="correlated")
     with _engine(p, nant, nf, prec="f8", data=data) as e:
