@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8", "f32v"])
     ap.add_argument("--xbatch", type=int, default=8, help="subints per X-engine launch")
     ap.add_argument("--no-fp32-compare", action="store_true")
+    ap.add_argument("--no-subbands", action="store_true", help="skip the eight-subband batch measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work (s); 0 disables")
     ap.add_argument("--seed", type=int, default=20261205)
     return ap.parse_args()
@@ -85,6 +86,46 @@ def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch):
         return (t1 - t0) / steps, per_step, nl, slots
     finally:
         e.close()
+
+
+def subband_batch(steps, warmup, seed, prec="f32"):
+    """Eight uneven subbands of a 16-antenna array, one 10 ms subint per step:
+    one hipGraph launch per subint (subbands as concurrent branches) against
+    plain launches one subband after another.  Synthetic random 2-bit input."""
+    from difx_tests_amd import FxBatch, synth
+    subs = workloads.C4
+    engines, iters = [], []
+    for i, w in enumerate(subs):
+        delays, fileoff = workloads.delay_model(w, seed + i)
+        data = synth.random_bytes(w.nant, w.subint_bytes, seed + i)
+        e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec=prec, max_ffts=w.nffts, max_bytes_per_ant=data.shape[1])
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        engines.append(e)
+        iters.append(np.arange(w.nffts, dtype=np.int32))
+    samples = sum(w.samples_per_subint for w in subs)
+    out = {"subbands": [{"nchan": w.nchan, "antennas": w.nant, "fs_msps": w.fs / 1e6, "ffts": w.nffts}
+                        for w in subs],
+           "xengine_prec": prec, "samples_per_subint": samples}
+    try:
+        with FxBatch(engines) as b:
+            for mode in ("launch", "graph"):
+                for _ in range(warmup):
+                    b.process(iters, 1, mode)
+                b.sync()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    b.process(iters, 1, mode)
+                b.sync()
+                dt = (time.perf_counter() - t0) / steps
+                out[mode] = {"ms_per_subint": round(dt * 1e3, 4), "gsamples_s": round(samples / dt / 1e9, 3)}
+            out["graph_speedup"] = round(out["launch"]["ms_per_subint"] / out["graph"]["ms_per_subint"], 3)
+            out["graph_captures"] = b.captures
+    finally:
+        for e in engines:
+            e.close()
+    return out
 
 
 def measured_traffic(workload, prec, xbatch, kernel):
@@ -196,6 +237,8 @@ def main():
             out[key] = {"xengine": p32, "value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
                         "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
                         "speedup_vs_this": round(value / v32, 3)}
+    if not a.no_subbands:
+        out["subband_batch"] = subband_batch(max(5, a.steps), 2, a.seed)
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(w, data, delays, fileoff, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -6,4 +6,4 @@ bindings (engine), the config grammar (config), the synthetic-data
 generator (synth), the workload definitions (workloads) and the build
 script (build).
 """
-from .engine import FxEngine, FxError, load_library  # noqa: F401
+from .engine import FxBatch, FxEngine, FxError, load_library  # noqa: F401

@@ -71,7 +71,9 @@ struct fx_context {
     float2* d_tap;
     int tap;
     int last_nffts;     // entries in the delay table of the last fx_fstage
+    int xsb;            // X-engine super-block size (0: by the signal count)
     int timing;
+    int capturing;      // launches are being recorded into a graph: no timing events
     std::vector<EvPair> pool;      // free event pairs
     std::vector<EvPair> pending;   // recorded, not yet summed
     double ms[TM_N];
@@ -280,10 +282,10 @@ extern "C" int fx_device_data(fx_context* c, void** dptr, long long* stride)
     return FX_OK;
 }
 
-static int ev_begin(fx_context* c, int kind, EvPair** out)
+static int ev_begin(fx_context* c, int kind, hipStream_t s, EvPair** out)
 {
     *out = nullptr;
-    if (!c->timing) return FX_OK;
+    if (!c->timing || c->capturing) return FX_OK;
     EvPair e;
     if (!c->pool.empty()) {
         e = c->pool.back();
@@ -293,25 +295,25 @@ static int ev_begin(fx_context* c, int kind, EvPair** out)
         HIPCHK(hipEventCreate(&e.b));
     }
     e.kind = kind;
-    HIPCHK(hipEventRecord(e.a, c->stream));
+    HIPCHK(hipEventRecord(e.a, s));
     c->pending.push_back(e);
     *out = &c->pending.back();
     return FX_OK;
 }
 
-static int ev_end(fx_context* c)
+static int ev_end(fx_context* c, hipStream_t s)
 {
-    if (!c->timing) return FX_OK;
-    HIPCHK(hipEventRecord(c->pending.back().b, c->stream));
+    if (!c->timing || c->capturing) return FX_OK;
+    HIPCHK(hipEventRecord(c->pending.back().b, s));
     return FX_OK;
 }
 
 static int ev_collect(fx_context* c)
 {
     if (c->pending.empty()) return FX_OK;
-    HIPCHK(hipStreamSynchronize(c->stream));
     for (EvPair& e : c->pending) {
         float ms = 0.0f;
+        HIPCHK(hipEventSynchronize(e.b));
         HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
         c->ms[e.kind] += ms;
         c->pool.push_back(e);
@@ -320,17 +322,23 @@ static int ev_collect(fx_context* c)
     return FX_OK;
 }
 
-#define TIMED(kind, launch)                      \
+#define TIMED(kind, s, launch)                   \
     do {                                         \
         EvPair* ep_;                             \
-        int rc_ = ev_begin(c, (kind), &ep_);     \
+        int rc_ = ev_begin(c, (kind), (s), &ep_); \
         if (rc_) return rc_;                     \
         HIPCHK(launch);                          \
-        rc_ = ev_end(c);                         \
+        rc_ = ev_end(c, (s));                    \
         if (rc_) return rc_;                     \
     } while (0)
 
-extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int split, int fft0, int* nslots)
+// Host-side checks of one F-stage call; allocates the tap buffer when needed
+// (never inside a graph capture).
+struct FStageCall {
+    int numffts, npad, maxoffset, fft0;
+};
+
+static int fstage_check(fx_context* c, int numffts, const int* iteration, int split, int fft0, FStageCall* f)
 {
     if (!c || !iteration) return fail(FX_EINVAL, "null argument");
     if (numffts < 1 || split < 1) return fail(FX_EINVAL, "numffts=%d split=%d", numffts, split);
@@ -348,13 +356,23 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
     for (int k = 0; k < numffts; k++)
         if (iteration[k] < 0 || (long long)iteration[k] * p.F > 0x7fffffffLL - p.F)
             return fail(FX_EINVAL, "iteration[%d]=%d out of range", k, iteration[k]);
-    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
     if (c->tap && !c->d_tap) {
         if (hipMalloc((void**)&c->d_tap, sizeof(float2) * (size_t)c->max_ffts * p.nant * 2 * p.nchan) != hipSuccess)
             return fail(FX_ENOMEM, "tap allocation failed");
     }
-    TIMED(TM_DELAY, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, numffts, npad, (int)maxoffset,
-                                 c->d_dtab, c->stream));
+    f->numffts = numffts;
+    f->npad = npad;
+    f->maxoffset = (int)maxoffset;
+    f->fft0 = fft0;
+    return FX_OK;
+}
+
+// Delay table, F-stage and counts on stream s, reading c->d_iter.
+static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
+{
+    const FxPlan& p = c->p;
+    TIMED(TM_DELAY, s, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, f.numffts, f.npad,
+                                    f.maxoffset, c->d_dtab, s));
     FStageArgs a;
     a.p = p;
     a.data = c->d_data;
@@ -365,24 +383,40 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
     a.stepchf = c->d_stepchf;
     a.out = c->d_xin;
     a.nfg = c->nfg;
-    a.fg0 = fft0 / c->G;
-    a.ngroups = npad / c->G;
+    a.fg0 = f.fft0 / c->G;
+    a.ngroups = f.npad / c->G;
     a.scale_inv = c->d_scale_inv;
     a.tap = c->tap ? c->d_tap : nullptr;
-    TIMED(TM_FSTAGE, launch_fstage(a, c->fprec, c->stream));
-    TIMED(TM_COUNTS, launch_counts(p.nant, npad, c->d_dtab, c->d_counts, c->stream));
-    if (c->timing) c->nl[0]++;
-    c->last_nffts = npad;
-    if (nslots) *nslots = npad;
+    TIMED(TM_FSTAGE, s, launch_fstage(a, c->fprec, s));
+    TIMED(TM_COUNTS, s, launch_counts(p.nant, f.npad, c->d_dtab, c->d_counts, s));
+    if (c->timing && !c->capturing) c->nl[0]++;
+    c->last_nffts = f.npad;
     return FX_OK;
 }
 
-extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
+extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int split, int fft0, int* nslots)
+{
+    FStageCall f;
+    int rc = fstage_check(c, numffts, iteration, split, fft0, &f);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+    rc = fstage_enqueue(c, f, c->stream);
+    if (rc) return rc;
+    if (nslots) *nslots = f.npad;
+    return FX_OK;
+}
+
+static int xstage_check(fx_context* c, int fft0, int nslots)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
     if (fft0 < 0 || nslots < 0 || fft0 % c->granule || nslots % c->granule || fft0 + nslots > c->max_ffts)
         return fail(FX_EINVAL, "bad slot range [%d,%d) (granule %d, capacity %d)", fft0, fft0 + nslots, c->granule,
                     c->max_ffts);
+    return FX_OK;
+}
+
+static int xstage_enqueue(fx_context* c, int fft0, int nslots, hipStream_t s)
+{
     if (nslots == 0) return FX_OK;
     XArgs a;
     a.p = c->p;
@@ -393,9 +427,17 @@ extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
     a.vacc = c->d_vacc;
     a.accumulate = 1;
     a.scale = c->d_scale;
-    TIMED(TM_XSTAGE, launch_xengine(a, c->prec, c->stream));
-    if (c->timing) c->nl[1]++;
+    a.sb = c->xsb;
+    TIMED(TM_XSTAGE, s, launch_xengine(a, c->prec, s));
+    if (c->timing && !c->capturing) c->nl[1]++;
     return FX_OK;
+}
+
+extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
+{
+    int rc = xstage_check(c, fft0, nslots);
+    if (rc) return rc;
+    return xstage_enqueue(c, fft0, nslots, c->stream);
 }
 
 extern "C" int fx_process(fx_context* c, int numffts, const int* iteration, int split)
@@ -531,3 +573,188 @@ extern "C" int fx_get_geometry(const fx_context* c, int* o)
     memcpy(o, v, sizeof(v));
     return FX_OK;
 }
+
+// --------------------------------------------------------------------------
+// Tuning options
+
+extern "C" int fx_set_option(fx_context* c, int option, int value)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    switch (option) {
+    case FX_OPT_XTILE:
+        if (value != 0 && value != 1 && value != 2 && value != 4)
+            return fail(FX_EINVAL, "X-engine tile %d: expected 0 (automatic), 1, 2 or 4", value);
+        c->xsb = value;
+        return FX_OK;
+    default: return fail(FX_EINVAL, "unknown option %d", option);
+    }
+}
+
+extern "C" int fx_get_option(const fx_context* c, int option, int* value)
+{
+    if (!c || !value) return fail(FX_EINVAL, "null argument");
+    switch (option) {
+    case FX_OPT_XTILE: *value = c->xsb > 0 ? c->xsb : xengine_super_blocks(c->p.nblk); return FX_OK;
+    default: return fail(FX_EINVAL, "unknown option %d", option);
+    }
+}
+
+// --------------------------------------------------------------------------
+// Batches of contexts (one per subband): one subint of every subband per
+// call, as one hipGraph launch whose branches are the subbands, or as plain
+// launches one subband after another.
+
+struct fx_batch {
+    std::vector<fx_context*> ctx;
+    hipStream_t s0 = nullptr;                 // launch stream
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> pre, join;        // per context
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<long long> key;               // call shape the graph was captured for
+    int captures = 0;
+};
+
+static void batch_drop_graph(fx_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->key.clear();
+}
+
+static void batch_free(fx_batch* b)
+{
+    if (b->s0) (void)hipStreamSynchronize(b->s0);
+    batch_drop_graph(b);
+    for (hipEvent_t e : b->pre) if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->join) if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->s0) (void)hipStreamDestroy(b->s0);
+    delete b;
+}
+
+extern "C" int fx_batch_create(fx_context* const* ctxs, int n, fx_batch** out)
+{
+    if (!ctxs || !out || n < 1) return fail(FX_EINVAL, "need at least one context");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return fail(FX_EINVAL, "context %d is null", i);
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return fail(FX_EINVAL, "context %d appears twice", i);
+    }
+    fx_batch* b = new fx_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->pre.assign(n, nullptr);
+    b->join.assign(n, nullptr);
+    bool ok = hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < n; i++)
+        ok = hipEventCreateWithFlags(&b->pre[i], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        batch_free(b);
+        return fail(FX_EHIP, "batch stream/event creation failed");
+    }
+    *out = b;
+    return FX_OK;
+}
+
+extern "C" void fx_batch_destroy(fx_batch* b)
+{
+    if (b) batch_free(b);
+}
+
+// Records the subint of every context into a graph on b->s0 (subbands as
+// branches forked from and joined back to s0).
+static int batch_capture(fx_batch* b, const std::vector<FStageCall>& calls)
+{
+    const int n = (int)b->ctx.size();
+    batch_drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal));
+    int rc = FX_OK;
+    hipError_t he = hipEventRecord(b->fork, b->s0);
+    for (int i = 0; he == hipSuccess && rc == FX_OK && i < n; i++) {
+        fx_context* c = b->ctx[i];
+        he = hipStreamWaitEvent(c->stream, b->fork, 0);
+        if (he != hipSuccess) break;
+        c->capturing = 1;
+        rc = fstage_enqueue(c, calls[i], c->stream);
+        if (rc == FX_OK) rc = xstage_enqueue(c, 0, calls[i].npad, c->stream);
+        c->capturing = 0;
+        if (rc == FX_OK) he = hipEventRecord(b->join[i], c->stream);
+        if (rc == FX_OK && he == hipSuccess) he = hipStreamWaitEvent(b->s0, b->join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->s0, &g);
+    if (rc != FX_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (he != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(FX_EHIP, "graph capture failed: %s", hipGetErrorString(he != hipSuccess ? he : ee));
+    }
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->captures++;
+    return FX_OK;
+}
+
+extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* const* iteration, int split, int mode)
+{
+    if (!b || !numffts || !iteration) return fail(FX_EINVAL, "null argument");
+    if (mode != FX_BATCH_GRAPH && mode != FX_BATCH_LAUNCH) return fail(FX_EINVAL, "bad mode %d", mode);
+    const int n = (int)b->ctx.size();
+    std::vector<FStageCall> calls(n);
+    std::vector<long long> key;
+    key.reserve(2 * n + 1);
+    for (int i = 0; i < n; i++) {
+        int rc = fstage_check(b->ctx[i], numffts[i], iteration[i], split, 0, &calls[i]);
+        if (rc) return fail(rc, "subband %d: %s", i, fx_last_error());
+        key.push_back(numffts[i]);
+        key.push_back((long long)b->ctx[i]->tap * 8 + b->ctx[i]->xsb);
+    }
+    key.push_back(split);
+    // s0 starts after the work already queued on every context's stream
+    for (int i = 0; i < n; i++) {
+        HIPCHK(hipEventRecord(b->pre[i], b->ctx[i]->stream));
+        HIPCHK(hipStreamWaitEvent(b->s0, b->pre[i], 0));
+    }
+    for (int i = 0; i < n; i++)
+        HIPCHK(hipMemcpyAsync(b->ctx[i]->d_iter, iteration[i], sizeof(int) * numffts[i], hipMemcpyHostToDevice,
+                              b->s0));
+    if (mode == FX_BATCH_GRAPH) {
+        if (!b->exec || key != b->key) {
+            int rc = batch_capture(b, calls);
+            if (rc) return rc;
+            b->key = key;
+        }
+        HIPCHK(hipGraphLaunch(b->exec, b->s0));
+    } else {
+        for (int i = 0; i < n; i++) {
+            int rc = fstage_enqueue(b->ctx[i], calls[i], b->s0);
+            if (rc == FX_OK) rc = xstage_enqueue(b->ctx[i], 0, calls[i].npad, b->s0);
+            if (rc) return rc;
+        }
+    }
+    // later calls on each context's own stream follow this subint
+    HIPCHK(hipEventRecord(b->done, b->s0));
+    for (int i = 0; i < n; i++) {
+        HIPCHK(hipStreamWaitEvent(b->ctx[i]->stream, b->done, 0));
+        b->ctx[i]->last_nffts = calls[i].npad;
+    }
+    return FX_OK;
+}
+
+extern "C" int fx_batch_sync(fx_batch* b)
+{
+    if (!b) return fail(FX_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(b->s0));
+    return FX_OK;
+}
+
+extern "C" int fx_batch_captures(const fx_batch* b) { return b ? b->captures : FX_EINVAL; }

@@ -30,6 +30,7 @@ struct XArgs {
     float2* vacc;                 // [nchan][nbp][32*32]
     int accumulate;               // 0: overwrite, 1: add
     const float* scale;           // FP8: [nchan][Spad] scales
+    int sb;                       // super-block size 1, 2 or 4; 0 picks by the block count
 };
 
 // F-stage: LG = log2(F); prec = FX_PREC_*; G is fixed per precision.
@@ -43,6 +44,7 @@ hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int l
 
 // X-engine; ngroups must be a multiple of xengine_chunk_groups(prec).
 int xengine_chunk_groups(int prec);
+int xengine_super_blocks(int nblk);
 hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s);
 hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s);
 hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, hipStream_t s);

@@ -125,7 +125,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const bool diag = SI == SJ;
 
     // blocks of this wave
-    int bI[NBW], bJ[NBW], nb = 0;
+    int bI[NBW], bJ[NBW];
+    unsigned okm = 0;                                     // bit t: slot t holds a block
     {
         const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
 #pragma unroll
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             const bool ok = q < nq && I < nblk && J < nblk;
             bI[t] = ok ? bi : 0;
             bJ[t] = ok ? bj : 0;
-            if (ok) nb = t + 1;
+            okm |= (unsigned)ok << t;
         }
     }
 
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             for (int ks = 0; ks < KC / 2; ks++) {
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
-                    if (t < nb) {
+                    if ((okm >> t) & 1u) {
                         const uint4 A = lds[stg][0][2 * ks + kg][bI[t] * FX_BLK + row];
                         const uint4 B = lds[stg][pb][2 * ks + kg][bJ[t] * FX_BLK + row];
                         const half8 hb = as_half8(B);
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             for (int g = 0; g < KC; g++) {
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
-                    if (t < nb) {
+                    if ((okm >> t) & 1u) {
                         const float4 A = as_float4(lds[stg][0][g][bI[t] * FX_BLK + row]);
                         const float4 B = as_float4(lds[stg][pb][g][bJ[t] * FX_BLK + row]);
                         const float av[2] = {kg ? A.z : A.x, kg ? A.w : A.y};
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             for (int ks = 0; ks < KC / 4; ks++) {
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
-                    if (t < nb) {
+                    if ((okm >> t) & 1u) {
                         const uint4 a0 = lds[stg][0][4 * ks + 2 * kg][bI[t] * FX_BLK + row];
                         const uint4 a1 = lds[stg][0][4 * ks + 2 * kg + 1][bI[t] * FX_BLK + row];
                         const uint4 b0 = lds[stg][pb][4 * ks + 2 * kg][bJ[t] * FX_BLK + row];
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const float* sc = (PREC == FX_PREC_F8) ? a.scale + (long long)c * Spad : nullptr;
 #pragma unroll
     for (int t = 0; t < NBW; t++) {
-        if (t < nb) {
+        if ((okm >> t) & 1u) {
             const int I = SI * SB + bI[t], J = SJ * SB + bJ[t];
             store_block(a, c, fx_bp_index(I, J, nblk), I, J, lane, accr[t], acci[t], sc);
         }
@@ -308,7 +309,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 
     for (int q0 = 0; q0 < nq; q0 += WAVES * NBV) {
         // this wave's blocks
-        int bI[NBV], bJ[NBV], nb = 0;
+        int bI[NBV], bJ[NBV];
+        unsigned okm = 0;
 #pragma unroll
         for (int t = 0; t < NBV; t++) {
             const int q = q0 + wave + t * WAVES;
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             const bool ok = q < nq && SI * SB + bi < nblk && SJ * SB + bj < nblk;
             bI[t] = ok ? bi : 0;
             bJ[t] = ok ? bj : 0;
-            if (ok) nb = t + 1;
+            okm |= (unsigned)ok << t;
         }
         float accr[NBV][16], acci[NBV][16];
 #pragma unroll
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             for (int g = 0; g < KC; g++) {
 #pragma unroll
                 for (int t = 0; t < NBV; t++) {
-                    if (t < nb) {
+                    if ((okm >> t) & 1u) {
                         const float4 B = lds[stg][pb][g][bJ[t] * FX_BLK + n];
 #pragma unroll
                         for (int v = 0; v < 16; v++) {
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         }
 #pragma unroll
         for (int t = 0; t < NBV; t++) {
-            if (t < nb) {
+            if ((okm >> t) & 1u) {
                 f32x16 re, im;
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
@@ -409,7 +411,7 @@ int xengine_super_blocks(int nblk) { return nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1; }
 template <int PREC>
 static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
 {
-    const int SB = xengine_super_blocks(a.p.nblk);
+    const int SB = a.sb > 0 ? a.sb : xengine_super_blocks(a.p.nblk);
     const int npan = (a.p.nblk + SB - 1) / SB;
     const int npairs = npan * (npan + 1) / 2;
     const dim3 grid(a.p.nchan * npairs), block(64 * WAVES);
@@ -428,12 +430,13 @@ hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s)
 {
     if (a.p.nchan <= 0 || a.ngroups <= 0) return hipSuccess;
     if (a.ngroups % xengine_chunk_groups(prec)) return hipErrorInvalidValue;   // callers pad to whole chunks
+    if (a.sb != 0 && a.sb != 1 && a.sb != 2 && a.sb != 4) return hipErrorInvalidValue;
     switch (prec) {
     case FX_PREC_F32: return launch_x_prec<FX_PREC_F32>(a, s);
     case FX_PREC_F16: return launch_x_prec<FX_PREC_F16>(a, s);
     case FX_PREC_F8:  return launch_x_prec<FX_PREC_F8>(a, s);
     case FX_PREC_F32V: {
-        const int SB = xengine_super_blocks(a.p.nblk);
+        const int SB = a.sb > 0 ? a.sb : xengine_super_blocks(a.p.nblk);
         const int npan = (a.p.nblk + SB - 1) / SB;
         const dim3 grid(a.p.nchan * (npan * (npan + 1) / 2)), block(64 * WAVES);
         switch (SB) {

@@ -22,8 +22,11 @@ NATIVE_SYMBOLS = [
     "fx_create", "fx_destroy", "fx_last_error", "fx_fft_granule", "fx_set_model", "fx_upload_data",
     "fx_upload_data_strided", "fx_device_data", "fx_fstage", "fx_xstage", "fx_process", "fx_reset",
     "fx_read_vis", "fx_read_delay_table", "fx_enable_tap", "fx_read_tap", "fx_set_fp8_scales", "fx_sync",
-    "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry",
+    "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry", "fx_set_option", "fx_get_option",
+    "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
 ]
+OPT_XTILE = 1
+BATCH_LAUNCH, BATCH_GRAPH = 0, 1
 # Every symbol include/kernel.h declares (kernel.h:61-134 of the reference).
 KERNELH_SYMBOLS = [
     "initLUT2bitReal", "init_FxKernel", "unpack", "getStationDelay", "unpackReal2bit", "fringerotate", "dofft",
@@ -82,6 +85,14 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_reset_timing.argtypes = [P]
     lib.fx_get_timing.argtypes = [P, P, P]
     lib.fx_get_geometry.argtypes = [P, P]
+    lib.fx_set_option.argtypes = [P, I, I]
+    lib.fx_get_option.argtypes = [P, I, C.POINTER(I)]
+    lib.fx_batch_create.argtypes = [P, I, C.POINTER(P)]
+    lib.fx_batch_destroy.argtypes = [P]
+    lib.fx_batch_destroy.restype = None
+    lib.fx_batch_process.argtypes = [P, P, P, I, I]
+    lib.fx_batch_sync.argtypes = [P]
+    lib.fx_batch_captures.argtypes = [P]
     _lib = lib
     return lib
 
@@ -189,6 +200,16 @@ class FxEngine:
         s = np.ascontiguousarray(scales, dtype=np.float32).reshape(self.nchan, 2 * self.nant)
         self._check(self.lib.fx_set_fp8_scales(self.h, _ptr(s)))
 
+    # ---- tuning
+    def set_xtile(self, sb: int) -> None:
+        """X-engine workgroup tile (1, 2 or 4 blocks per panel side; 0: automatic)."""
+        self._check(self.lib.fx_set_option(self.h, OPT_XTILE, sb))
+
+    def xtile(self) -> int:
+        v = C.c_int()
+        self._check(self.lib.fx_get_option(self.h, OPT_XTILE, C.byref(v)))
+        return v.value
+
     # ---- timing
     def set_timing(self, on: bool = True) -> None:
         self._check(self.lib.fx_set_timing(self.h, 1 if on else 0))
@@ -202,3 +223,53 @@ class FxEngine:
         nl = np.zeros(2, np.int32)
         self._check(self.lib.fx_get_timing(self.h, _ptr(ms), _ptr(nl)))
         return dict(zip(("delay", "fstage", "counts", "xstage"), ms.tolist())), nl.tolist()
+
+
+class FxBatch:
+    """Several engines (one per subband) run one subint at a time, as one
+    hipGraph launch (mode "graph") or as plain launches one subband after
+    another (mode "launch")."""
+
+    def __init__(self, engines):
+        self.lib = load_library()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        self._check(self.lib.fx_batch_create(arr, len(self.engines), C.byref(h)))
+        self.h = h
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise FxError(f"libdifxfx error {rc}: {self.lib.fx_last_error().decode()}")
+
+    def process(self, iterations, split: int = 1, mode: str = "graph") -> None:
+        its = [np.ascontiguousarray(i, dtype=np.int32) for i in iterations]
+        assert len(its) == len(self.engines)
+        n = np.array([len(i) for i in its], np.int32)
+        ptrs = (C.c_void_p * len(its))(*[i.ctypes.data for i in its])
+        m = BATCH_GRAPH if mode == "graph" else BATCH_LAUNCH
+        self._check(self.lib.fx_batch_process(self.h, _ptr(n), ptrs, split, m))
+
+    def sync(self) -> None:
+        self._check(self.lib.fx_batch_sync(self.h))
+
+    @property
+    def captures(self) -> int:
+        return int(self.lib.fx_batch_captures(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.fx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

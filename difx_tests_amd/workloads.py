@@ -108,13 +108,15 @@ def make_subint(w: Workload, seed: int, nffts: int | None = None):
     return data, delays, fileoff, np.arange(nf, dtype=np.int32)
 
 
-# The five workloads of the headline benchmark (BASELINE configs 0..4).
+# The benchmark workloads: plain FP32 at 4 antennas, FP32 at 16 antennas with
+# fringe rates and fractional delays, FP16 at 64 antennas, FP8 at 256 antennas.
 C1 = Workload("c1-4ant-1024ch", nant=4, nchan=1024, fs=128e6, rate_max=1e-9)
 C2 = Workload("c2-16ant-2048ch", nant=16, nchan=2048, fs=128e6, integration_s=0.1, fringe_hz=50.0,
               frac_delay=True)
 C3 = Workload("c3-64ant-4096ch", nant=64, nchan=4096, fs=128e6, integration_s=1.0, rate_max=3e-6, prec="f16")
 C5 = Workload("c5-256ant-1024ch", nant=256, nchan=1024, fs=64e6, integration_s=1.0, rate_max=1e-7, prec="f8")
-# C4: eight uneven subbands (channels, sample rate, antennas)
+# Eight uneven subbands (channels, sample rate, antennas) of a 16-antenna array,
+# each correlating its own subset of the antennas.
 C4_SUBBANDS = [(256, 16e6, 8), (512, 32e6, 10), (1024, 64e6, 12), (2048, 128e6, 16), (4096, 256e6, 16),
                (8192, 256e6, 14), (16384, 256e6, 16), (1024, 128e6, 9)]
 C4 = [Workload(f"c4-sb{i}-{n}ch", nant=a, nchan=n, fs=fs) for i, (n, fs, a) in enumerate(C4_SUBBANDS)]

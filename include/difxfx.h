@@ -56,7 +56,8 @@ int fx_create(const fx_params *prm, fx_context **out);
 void fx_destroy(fx_context *ctx);
 const char *fx_last_error(void);
 
-/* FFT slots are grouped: fx_fstage writes whole K steps of this many FFTs. */
+/* FFT slots are grouped: fx_fstage writes whole X-engine K chunks of this
+ * many FFTs. */
 int fx_fft_granule(const fx_context *ctx);
 
 /* Delay model: delays[3*a + {0,1,2}] = c0, c1, c2 (seconds, t in FFT units;
@@ -117,6 +118,34 @@ int fx_get_timing(fx_context *ctx, double *ms4, int *nlaunch2);
 /* Geometry used by the kernels (for reports): fills
  * {F, stride, sub, nstep, nfstep, S, Spad, nblk, nbp, G, granule}. */
 int fx_get_geometry(const fx_context *ctx, int *out11);
+
+/* Tuning options (the autotuner sets these; results do not depend on them).
+ * FX_OPT_XTILE: X-engine workgroup tile, a square of 1, 2 or 4 blocks of
+ * 32x32 signals per panel; 0 selects by the signal count. */
+#define FX_OPT_XTILE 1
+int fx_set_option(fx_context *ctx, int option, int value);
+int fx_get_option(const fx_context *ctx, int option, int *value);
+
+/* Batches: several contexts (one per subband, each with its own antennas,
+ * channels, bandwidth and precision) processed one subint at a time.
+ * fx_batch_process runs fx_process(ctx[i], numffts[i], iteration[i], split)
+ * for every context:
+ *   FX_BATCH_GRAPH   as one hipGraph launch per call, the subbands as
+ *                    concurrent branches; the graph is recorded on the first
+ *                    call and again only when numffts, split or a context's
+ *                    options change
+ *   FX_BATCH_LAUNCH  as plain launches, one subband after another
+ * Work queued earlier on a context (uploads, fx_reset) runs first; later calls
+ * on a context run after the subint.  Results are identical in both modes. */
+typedef struct fx_batch fx_batch;
+#define FX_BATCH_LAUNCH 0
+#define FX_BATCH_GRAPH 1
+int fx_batch_create(fx_context *const *ctxs, int n, fx_batch **out);
+void fx_batch_destroy(fx_batch *batch);
+int fx_batch_process(fx_batch *batch, const int *numffts, const int *const *iteration, int split, int mode);
+int fx_batch_sync(fx_batch *batch);
+/* Number of graph recordings so far (for tests). */
+int fx_batch_captures(const fx_batch *batch);
 
 #ifdef __cplusplus
 }
