@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3-64ant-4096ch", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8", "f32v"])
-    ap.add_argument("--xbatch", type=int, default=8, help="subints per X-engine launch")
+    ap.add_argument("--xbatch", type=int, default=0, help="subints per X-engine launch (0: the autotuner's choice)")
+    ap.add_argument("--no-autotune", action="store_true", help="default radix plan and X-engine tile, xbatch 8")
     ap.add_argument("--no-fp32-compare", action="store_true")
     ap.add_argument("--no-subbands", action="store_true", help="skip the eight-subband batch measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work (s); 0 disables")
@@ -53,14 +54,22 @@ def parse():
     return ap.parse_args()
 
 
-def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch):
-    """Returns (seconds per step, kernel ms per step dict, launches)."""
+MAX_XBATCH = 16
+
+
+def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune=True):
+    """Returns (seconds per step, kernel ms per step dict, launch counts, slots
+    per subint, tuning dict).  xbatch 0 takes the autotuner's choice."""
     nf = len(it)
-    e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec=prec, max_ffts=xbatch * (nf + 64),
-                 max_bytes_per_ant=data.shape[1])
+    cap = (xbatch or MAX_XBATCH) * (nf + 64)
+    e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec=prec, max_ffts=cap, max_bytes_per_ant=data.shape[1])
     try:
         e.set_model(delays, fileoff)
         e.upload(data)
+        tuned = e.autotune(nf, reps=3) if tune else {}
+        if not xbatch:
+            xbatch = tuned.get("xbatch", 8)
+        tuned["xbatch_used"] = xbatch
         gran = e.granule
         slots = (nf + gran - 1) // gran * gran
 
@@ -83,12 +92,12 @@ def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch):
         ms, nl = e.timing()
         e.set_timing(False)
         per_step = {k: v / steps for k, v in ms.items()}
-        return (t1 - t0) / steps, per_step, nl, slots
+        return (t1 - t0) / steps, per_step, nl, slots, tuned
     finally:
         e.close()
 
 
-def subband_batch(steps, warmup, seed, prec="f32"):
+def subband_batch(steps, warmup, seed, prec="f32", tune=True):
     """Eight uneven subbands of a 16-antenna array, one 10 ms subint per step:
     one hipGraph launch per subint (subbands as concurrent branches) against
     plain launches one subband after another.  Synthetic random 2-bit input."""
@@ -121,6 +130,22 @@ def subband_batch(steps, warmup, seed, prec="f32"):
                 dt = (time.perf_counter() - t0) / steps
                 out[mode] = {"ms_per_subint": round(dt * 1e3, 4), "gsamples_s": round(samples / dt / 1e9, 3)}
             out["graph_speedup"] = round(out["launch"]["ms_per_subint"] / out["graph"]["ms_per_subint"], 3)
+            if tune:
+                choices = []
+                for e, w in zip(engines, subs):
+                    r = e.autotune(w.nffts, reps=3)
+                    choices.append({"plan": r["plan"], "xtile": r["xtile"]})
+                    e.reset()
+                for _ in range(warmup):
+                    b.process(iters, 1, "graph")
+                b.sync()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    b.process(iters, 1, "graph")
+                b.sync()
+                dt = (time.perf_counter() - t0) / steps
+                out["graph_autotuned"] = {"ms_per_subint": round(dt * 1e3, 4),
+                                          "gsamples_s": round(samples / dt / 1e9, 3), "choices": choices}
             out["graph_captures"] = b.captures
     finally:
         for e in engines:
@@ -181,25 +206,27 @@ def main():
     prec = a.prec or w.prec
     data, delays, fileoff, it = workloads.make_subint(w, a.seed)
     nf = len(it)
-    samples_per_step = a.xbatch * w.samples_per_subint
-
-    sec, kms, nl, slots = run_pipeline(w, prec, data, delays, fileoff, it, a.steps, a.warmup, a.xbatch)
+    tune = not a.no_autotune
+    xb_req = a.xbatch or (0 if tune else 8)
+    sec, kms, nl, slots, tuned = run_pipeline(w, prec, data, delays, fileoff, it, a.steps, a.warmup, xb_req, tune)
+    xbatch = tuned["xbatch_used"]
+    samples_per_step = xbatch * w.samples_per_subint
     value = samples_per_step / sec / 1e9
 
     # algorithmic work per launch
     f_launch_bytes = w.samples_per_subint * (0.25 + OUT_BYTES_PER_SAMPLE[prec])
-    x_flops = 8.0 * 4 * w.nbl_with_autos * w.nchan * nf * a.xbatch
-    f_ms = kms["fstage"] / a.xbatch          # per F launch
+    x_flops = 8.0 * 4 * w.nbl_with_autos * w.nchan * nf * xbatch
+    f_ms = kms["fstage"] / xbatch            # per F launch
     x_ms = kms["xstage"]                      # one X launch per step
     f_gbs = f_launch_bytes / (f_ms * 1e-3) / 1e9
     x_tflops = x_flops / (x_ms * 1e-3) / 1e12
     if kms["fstage"] >= kms["xstage"]:
-        traffic, src = measured_traffic(w.name, prec, a.xbatch, "fx_fstage_kernel")
+        traffic, src = measured_traffic(w.name, prec, xbatch, "fx_fstage_kernel")
         roof = {"kernel": "fx_fstage_kernel", "bound": "hbm", "achieved": round(f_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": int(f_launch_bytes), "avg_launch_ms": round(f_ms, 4)}
     else:
-        traffic, src = measured_traffic(w.name, prec, a.xbatch, "fx_xengine_kernel")
+        traffic, src = measured_traffic(w.name, prec, xbatch, "fx_xengine_kernel")
         pk = MFMA_PEAK_TFLOPS[prec]
         roof = {"kernel": "fx_xengine_kernel", "bound": "mfma", "achieved": round(x_tflops, 2), "peak": pk,
                 "unit": "TFLOP/s", "frac": round(x_tflops / pk, 4), "traffic": traffic,
@@ -221,24 +248,26 @@ def main():
         "dtype": prec,
         "data": "synthetic",
         "config": {"workload": w.name, "antennas": w.nant, "pols": 2, "nchan": w.nchan, "fs_msps": w.fs / 1e6,
-                   "subint_ms": w.subint_s * 1e3, "ffts_per_subint": nf, "subints_per_step": a.xbatch,
+                   "subint_ms": w.subint_s * 1e3, "ffts_per_subint": nf, "subints_per_step": xbatch,
                    "xengine_prec": prec, "parallelism": "single GPU"},
         "xengine_tflops": round(x_tflops, 2),
         "fstage_gbs": round(f_gbs, 1),
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kms.items()},
-        "realtime_factor": round((a.xbatch * w.subint_s) / sec, 2),
+        "realtime_factor": round((xbatch * w.subint_s) / sec, 2),
+        "autotune": tuned if tune else None,
         "roofline": roof,
     }
     if not a.no_fp32_compare and prec not in ("f32", "f32v"):
         # the plain FP32 GPU pipeline (vector-ALU X-engine) and the FP32 MFMA one
         for key, p32 in (("fp32_pipeline", "f32v"), ("fp32_mfma_pipeline", "f32")):
-            s32, k32, _, _ = run_pipeline(w, p32, data, delays, fileoff, it, max(3, a.steps // 2), 1, a.xbatch)
+            s32, k32, _, _, t32 = run_pipeline(w, p32, data, delays, fileoff, it, max(3, a.steps // 2), 1, xbatch,
+                                               tune)
             v32 = samples_per_step / s32 / 1e9
             out[key] = {"xengine": p32, "value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
                         "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
-                        "speedup_vs_this": round(value / v32, 3)}
+                        "autotune": t32 if tune else None, "speedup_vs_this": round(value / v32, 3)}
     if not a.no_subbands:
-        out["subband_batch"] = subband_batch(max(5, a.steps), 2, a.seed)
+        out["subband_batch"] = subband_batch(max(5, a.steps), 2, a.seed, tune=tune)
     if a.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(w, data, delays, fileoff, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "difxfx.h"
@@ -72,6 +73,8 @@ struct fx_context {
     int tap;
     int last_nffts;     // entries in the delay table of the last fx_fstage
     int xsb;            // X-engine super-block size (0: by the signal count)
+    int fplan;          // F-stage radix plan
+    int xbatch;         // subints per X-engine launch recommended to callers (1 unless tuned)
     int timing;
     int capturing;      // launches are being recorded into a graph: no timing events
     std::vector<EvPair> pool;      // free event pairs
@@ -112,6 +115,8 @@ extern "C" void fx_destroy(fx_context* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     free_ctx(ctx);
 }
+
+static void tune_apply_cached(fx_context* c);
 
 // Host-side plan: kernel.c:13-39, generalised to power-of-two nchan.
 static int make_plan(const fx_params* prm, FxPlan* p)
@@ -162,6 +167,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     FxPlan& p = c->p;
     c->prec = prm->prec;
     c->fprec = prm->prec == FX_PREC_F32V ? FX_PREC_F32 : prm->prec;   // F-stage output format
+    c->xbatch = 1;
     c->G = fx_fstage_group_size(c->fprec);
     c->kgroups = xengine_chunk_groups(prm->prec);
     c->granule = c->G * c->kgroups;
@@ -234,6 +240,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
         free_ctx(c);
         return fail(FX_EHIP, "context initialisation failed");
     }
+    tune_apply_cached(c);
     *out = c;
     return FX_OK;
 }
@@ -382,11 +389,12 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     a.subchf = c->d_subchf;
     a.stepchf = c->d_stepchf;
     a.out = c->d_xin;
-    a.nfg = c->nfg;
+    a.kc = c->kgroups;
     a.fg0 = f.fft0 / c->G;
     a.ngroups = f.npad / c->G;
     a.scale_inv = c->d_scale_inv;
     a.tap = c->tap ? c->d_tap : nullptr;
+    a.plan = c->fplan;
     TIMED(TM_FSTAGE, s, launch_fstage(a, c->fprec, s));
     TIMED(TM_COUNTS, s, launch_counts(p.nant, f.npad, c->d_dtab, c->d_counts, s));
     if (c->timing && !c->capturing) c->nl[0]++;
@@ -421,7 +429,6 @@ static int xstage_enqueue(fx_context* c, int fft0, int nslots, hipStream_t s)
     XArgs a;
     a.p = c->p;
     a.xin = c->d_xin;
-    a.nfg = c->nfg;
     a.fg0 = fft0 / c->G;
     a.ngroups = nslots / c->G;
     a.vacc = c->d_vacc;
@@ -586,6 +593,15 @@ extern "C" int fx_set_option(fx_context* c, int option, int value)
             return fail(FX_EINVAL, "X-engine tile %d: expected 0 (automatic), 1, 2 or 4", value);
         c->xsb = value;
         return FX_OK;
+    case FX_OPT_FPLAN:
+        if (value < 0 || value >= fstage_plans(c->p.lg))
+            return fail(FX_EINVAL, "F-stage plan %d: %d plans at F=%d", value, fstage_plans(c->p.lg), c->p.F);
+        c->fplan = value;
+        return FX_OK;
+    case FX_OPT_XBATCH:
+        if (value < 1) return fail(FX_EINVAL, "subints per X-engine launch must be >= 1");
+        c->xbatch = value;
+        return FX_OK;
     default: return fail(FX_EINVAL, "unknown option %d", option);
     }
 }
@@ -595,6 +611,9 @@ extern "C" int fx_get_option(const fx_context* c, int option, int* value)
     if (!c || !value) return fail(FX_EINVAL, "null argument");
     switch (option) {
     case FX_OPT_XTILE: *value = c->xsb > 0 ? c->xsb : xengine_super_blocks(c->p.nblk); return FX_OK;
+    case FX_OPT_FPLAN: *value = c->fplan; return FX_OK;
+    case FX_OPT_NPLANS: *value = fstage_plans(c->p.lg); return FX_OK;
+    case FX_OPT_XBATCH: *value = c->xbatch; return FX_OK;
     default: return fail(FX_EINVAL, "unknown option %d", option);
     }
 }
@@ -716,7 +735,7 @@ extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* cons
         int rc = fstage_check(b->ctx[i], numffts[i], iteration[i], split, 0, &calls[i]);
         if (rc) return fail(rc, "subband %d: %s", i, fx_last_error());
         key.push_back(numffts[i]);
-        key.push_back((long long)b->ctx[i]->tap * 8 + b->ctx[i]->xsb);
+        key.push_back(((long long)b->ctx[i]->tap * 8 + b->ctx[i]->xsb) * 8 + b->ctx[i]->fplan);
     }
     key.push_back(split);
     // s0 starts after the work already queued on every context's stream
@@ -758,3 +777,200 @@ extern "C" int fx_batch_sync(fx_batch* b)
 }
 
 extern "C" int fx_batch_captures(const fx_batch* b) { return b ? b->captures : FX_EINVAL; }
+
+// --------------------------------------------------------------------------
+// Autotuner.  Candidates: every F-stage radix plan, every X-engine tile that
+// fits the signal count, and 1, 2, 4, ... subints per X-engine launch up to
+// the context's FFT capacity.  Each candidate runs once untimed and `reps`
+// times between HIP events on the context's stream.  The cache is a text
+// file, one line per (arch, nant, nchan, prec):
+//   arch nant nchan prec plan xtile xbatch f_ms x_ms_per_subint
+
+struct TuneEntry {
+    std::string arch;
+    int nant, nchan, prec, plan, xtile, xbatch;
+    double fms, xms;
+};
+
+static std::string device_arch()
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    const size_t colon = a.find(':');
+    return colon == std::string::npos ? a : a.substr(0, colon);
+}
+
+static std::vector<TuneEntry> tune_read(const char* path)
+{
+    std::vector<TuneEntry> v;
+    FILE* f = path ? fopen(path, "r") : nullptr;
+    if (!f) return v;
+    char line[512], arch[128];
+    while (fgets(line, sizeof(line), f)) {
+        TuneEntry e;
+        if (line[0] == '#') continue;
+        if (sscanf(line, "%127s %d %d %d %d %d %d %lf %lf", arch, &e.nant, &e.nchan, &e.prec, &e.plan, &e.xtile,
+                   &e.xbatch, &e.fms, &e.xms) == 9) {
+            e.arch = arch;
+            v.push_back(e);
+        }
+    }
+    fclose(f);
+    return v;
+}
+
+static int tune_write(const char* path, const TuneEntry& e)
+{
+    std::vector<TuneEntry> v = tune_read(path);
+    bool found = false;
+    for (TuneEntry& x : v)
+        if (x.arch == e.arch && x.nant == e.nant && x.nchan == e.nchan && x.prec == e.prec) {
+            x = e;
+            found = true;
+        }
+    if (!found) v.push_back(e);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return fail(FX_EINVAL, "cannot write tuning cache %s", tmp.c_str());
+    fprintf(f, "# arch nant nchan prec plan xtile xbatch f_ms x_ms_per_subint\n");
+    for (const TuneEntry& x : v)
+        fprintf(f, "%s %d %d %d %d %d %d %.6f %.6f\n", x.arch.c_str(), x.nant, x.nchan, x.prec, x.plan, x.xtile,
+                x.xbatch, x.fms, x.xms);
+    const bool ok = fclose(f) == 0 && rename(tmp.c_str(), path) == 0;
+    return ok ? FX_OK : fail(FX_EINVAL, "cannot write tuning cache %s", path);
+}
+
+static void tune_apply_cached(fx_context* c)
+{
+    const char* path = getenv("FX_TUNE_CACHE");
+    if (!path || !*path) return;
+    const std::string arch = device_arch();
+    for (const TuneEntry& e : tune_read(path))
+        if (e.arch == arch && e.nant == c->p.nant && e.nchan == c->p.nchan && e.prec == c->prec) {
+            if (e.plan >= 0 && e.plan < fstage_plans(c->p.lg)) c->fplan = e.plan;
+            if (e.xtile == 1 || e.xtile == 2 || e.xtile == 4) c->xsb = e.xtile;
+            if (e.xbatch >= 1) c->xbatch = e.xbatch;
+        }
+}
+
+// Average ms of `reps` runs of fn (after one untimed run) on c->stream.
+template <class Fn>
+static int tune_time(fx_context* c, int reps, double* ms, Fn fn)
+{
+    int rc = fn();
+    if (rc) return rc;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(FX_EHIP, "hipEventCreate failed");
+    }
+    hipError_t he = hipEventRecord(a, c->stream);
+    for (int r = 0; rc == FX_OK && he == hipSuccess && r < reps; r++) rc = fn();
+    if (he == hipSuccess) he = hipEventRecord(b, c->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(b);
+    float t = 0.0f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (he != hipSuccess) return fail(FX_EHIP, "tuning timer: %s", hipGetErrorString(he));
+    *ms = t / reps;
+    return FX_OK;
+}
+
+extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cache_path, double* report)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    if (reps < 1) return fail(FX_EINVAL, "reps=%d", reps);
+    std::vector<int> it(numffts > 0 ? numffts : 1);
+    for (int k = 0; k < numffts; k++) it[k] = k;
+    FStageCall f;
+    int rc = fstage_check(c, numffts, it.data(), 1, 0, &f);
+    if (rc) return rc;
+    const int saved_timing = c->timing, saved_tap = c->tap;
+    c->timing = 0;
+    c->tap = 0;
+    HIPCHK(hipMemcpyAsync(c->d_iter, it.data(), sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+
+    // F-stage radix plans
+    double best_f = 1e30;
+    int best_plan = 0;
+    for (int v = 0; rc == FX_OK && v < fstage_plans(c->p.lg); v++) {
+        c->fplan = v;
+        double ms = 0.0;
+        rc = tune_time(c, reps, &ms, [&] { return fstage_enqueue(c, f, c->stream); });
+        if (rc == FX_OK && ms < best_f) {
+            best_f = ms;
+            best_plan = v;
+        }
+    }
+    c->fplan = best_plan;
+    // X-engine tiles over one subint
+    double best_x = 1e30;
+    int best_tile = 1;
+    const int tiles[3] = {1, 2, 4};
+    for (int t = 0; rc == FX_OK && t < 3; t++) {
+        if (tiles[t] > 1 && c->p.nblk < tiles[t] - 1) continue;       // tile far larger than the signal count
+        c->xsb = tiles[t];
+        double ms = 0.0;
+        rc = tune_time(c, reps, &ms, [&] { return xstage_enqueue(c, 0, f.npad, c->stream); });
+        if (rc == FX_OK && ms < best_x) {
+            best_x = ms;
+            best_tile = tiles[t];
+        }
+    }
+    c->xsb = best_tile;
+    // subints per X-engine launch: whole steps of k F-stages and one X-engine
+    // launch over their slots; the smallest k within 3% of the best time per
+    // subint
+    double per_sub[8];
+    int counts[8], nk = 0;
+    for (int k = 1; rc == FX_OK && nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) {
+        double ms = 0.0;
+        rc = tune_time(c, reps, &ms, [&] {
+            for (int s = 0; s < k; s++) {
+                FStageCall fs = f;
+                fs.fft0 = s * f.npad;
+                const int r = fstage_enqueue(c, fs, c->stream);
+                if (r) return r;
+            }
+            return xstage_enqueue(c, 0, k * f.npad, c->stream);
+        });
+        counts[nk] = k;
+        per_sub[nk++] = ms / k;
+    }
+    int best_k = 1;
+    if (rc == FX_OK && nk > 0) {
+        double m = per_sub[0];
+        for (int i = 1; i < nk; i++) m = per_sub[i] < m ? per_sub[i] : m;
+        for (int i = 0; i < nk; i++)
+            if (per_sub[i] <= 1.03 * m) {
+                best_k = counts[i];
+                break;
+            }
+    }
+    c->xbatch = best_k;
+    c->timing = saved_timing;
+    c->tap = saved_tap;
+    if (rc) return rc;
+    // the trials accumulated into the visibilities: start clean
+    HIPCHK(launch_zero(c->d_vacc, c->vacc_bytes, c->stream));
+    HIPCHK(launch_zero(c->d_counts, sizeof(int) * c->p.nant * c->p.nant, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (report) {
+        report[0] = best_plan;
+        report[1] = best_tile;
+        report[2] = best_k;
+        report[3] = best_f;
+        report[4] = best_x;
+    }
+    const char* path = cache_path ? cache_path : getenv("FX_TUNE_CACHE");
+    if (path && *path) {
+        TuneEntry e{device_arch(), c->p.nant, c->p.nchan, c->prec, best_plan, best_tile, best_k, best_f, best_x};
+        return tune_write(path, e);
+    }
+    return FX_OK;
+}

@@ -229,18 +229,29 @@ __device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __rest
 // butterfly; at F = 8192 two blocks (79 KB of LDS each) share a CU, so one
 // block's barriers and window loads overlap the other's arithmetic.
 
-template <int LG> struct FPlan;
-template <> struct FPlan<9>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1; };
-template <> struct FPlan<10> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1; };
-template <> struct FPlan<11> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1; };
-template <> struct FPlan<12> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
-template <> struct FPlan<13> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2; };
-template <> struct FPlan<14> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4; };
+// Plan V = 0 and, where a second one exists, V = 1; the autotuner picks per
+// context (fx_autotune).
+template <int LG, int V = 0> struct FPlan;
+template <> struct FPlan<9, 0>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1; };
+template <> struct FPlan<10, 0> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1; };
+template <> struct FPlan<10, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 16, R3 = 1; };
+template <> struct FPlan<11, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1; };
+template <> struct FPlan<11, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 16, R2 = 16, R3 = 1; };
+template <> struct FPlan<12, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
+template <> struct FPlan<12, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 8; };
+template <> struct FPlan<13, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2; };
+template <> struct FPlan<13, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 16, R2 = 16, R3 = 4; };
+template <> struct FPlan<14, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4; };
+template <> struct FPlan<14, 1> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 8,  R3 = 8; };
+
+// number of plans per FFT length
+__host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : 2; }
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <int LG> struct Geo {
-    using PL_ = FPlan<LG>;
+template <int LG, int V = 0> struct Geo {
+    using PL_ = FPlan<LG, V>;
+    static constexpr int LGV = LG;
     static constexpr int F = 1 << LG;
     static constexpr int NCH = F / 2;
     static constexpr int STRIDE = 1 << ((LG - 1) / 2);     // 2^floor(log2(nchan)/2)
@@ -255,8 +266,9 @@ template <int LG> struct Geo {
     static constexpr int PL = F / RL;                       // transform length before the last pass
     static constexpr int NOUT = NCH / NT;                   // channels per thread per FFT
     static constexpr int FP = F + F / 16;                   // padded LDS row
-    // waves per SIMD the LDS allows at F = 8192 (two 8-wave blocks per CU)
-    static constexpr int WPE = LG >= 13 ? 4 : 3;
+    // waves per SIMD: at F >= 8192 the LDS allows two 8-wave blocks (or one
+    // 16-wave block) per CU
+    static constexpr int WPE = (LG >= 13 || (V == 1 && LG >= 12)) ? 4 : 3;
     static_assert(R0 * R1 * R2 * R3 == F, "plan must factor F");
     static_assert((F / R0) % SUB == 0, "pass-0 rotator reuse needs F/R0 to be a multiple of sub");
     static_assert(R0 == 16 || (R0 == 8 && F / R0 >= 64), "pass-0 writes pad(i*R0 + r) = pad(i*R0) + r");
@@ -338,10 +350,9 @@ int fx_fstage_group_size(int prec)
 
 // TWS is the stride of the twiddle table: 1 when tw holds exp(-2 pi i k / 2^LG),
 // 2 when a 2^LG-point sub-transform reads the table of a 2^(LG+1)-point one.
-template <int LG, int R, int P, bool TW_LDS, int TWS = 1>
+template <class Gm, int R, int P, bool TW_LDS, int TWS = 1>
 __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const float2* __restrict__ tw, int tid)
 {
-    using Gm = Geo<LG>;
     constexpr int F = Gm::F, NT = Gm::NT, FP = Gm::FP;
     constexpr int T = F / R, PER = T / NT;
     static_assert(PER * NT == T, "plan/thread mismatch");
@@ -386,11 +397,11 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
 // The fused F-stage kernel: one block = one antenna, one polarisation, G
 // consecutive FFTs.
 
-template <int LG, int PREC, bool TAP>
-__global__ __launch_bounds__(Geo<LG>::NT) __attribute__((amdgpu_waves_per_eu(Geo<LG>::WPE)))
+template <int LG, int V, int PREC, bool TAP>
+__global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_eu(Geo<LG, V>::WPE)))
 void fx_fstage_kernel(FStageArgs args)
 {
-    using Gm = Geo<LG>;
+    using Gm = Geo<LG, V>;
     constexpr int F = Gm::F, NCH = Gm::NCH, NT = Gm::NT, FP = Gm::FP;
     constexpr int STRIDE = Gm::STRIDE, SUB = Gm::SUB, NSTEP = Gm::NSTEP, NFSTEP = Gm::NFSTEP;
     constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2, RL = Gm::RL, PL = Gm::PL;
@@ -422,6 +433,9 @@ void fx_fstage_kernel(FStageArgs args)
     const int ant = (lid >> 1) - grp * p.nant;
     const int fgo = args.fg0 + grp;
     const int sig = 2 * ant + pol;
+    // X input row of (channel 0, this group, this signal); channel c adds c*kc*Spad
+    const int kq = fgo / args.kc;
+    const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
     int tid = threadIdx.x;
 
     for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
@@ -536,8 +550,8 @@ void fx_fstage_kernel(FStageArgs args)
         __syncthreads();
 
         // ---- middle passes
-        mid_pass<LG, R1, R0, true>(buf, tw1, args.tw, tid);
-        if constexpr (Gm::NP == 4) mid_pass<LG, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
+        mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
+        if constexpr (Gm::NP == 4) mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
 
         // ---- last pass: radix RL, channels < nchan only, frac-sample rotation
 #pragma unroll
@@ -562,7 +576,7 @@ void fx_fstage_kernel(FStageArgs args)
         }
     }
 
-    // ---- stores: corner-turned [chan][group][Spad][G]
+    // ---- stores: corner-turned [chunk][chan][group in chunk][Spad] rows of G FFTs
     tid = threadIdx.x;
 #pragma unroll
     for (int m = 0; m < PERL; m++) {
@@ -571,7 +585,7 @@ void fx_fstage_kernel(FStageArgs args)
         for (int r = 0; r < RL / 2; r++) {
             const int c = i + r * PL;
             const int ou = m * (RL / 2) + r;
-            const long long row = ((long long)c * args.nfg + fgo) * p.Spad + sig;
+            const long long row = xrow0 + (long long)c * args.kc * p.Spad;
             if constexpr (PREC == FX_PREC_F16) {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
 #pragma unroll
@@ -623,11 +637,11 @@ __device__ __forceinline__ void store_slot(const FStageArgs& args, long long row
     }
 }
 
-template <int PREC, bool TAP>
-__global__ __launch_bounds__(Geo<kEoLG - 1>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+template <int V, int PREC, bool TAP>
+__global__ __launch_bounds__((Geo<kEoLG - 1, V>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
 void fx_fstage_eo_kernel(FStageArgs args)
 {
-    using Gh = Geo<kEoLG - 1>;                                  // geometry of one half
+    using Gh = Geo<kEoLG - 1, V>;                               // geometry of one half
     constexpr int LG = kEoLG, F = 1 << LG, NCH = F / 2, H = F / 2;
     constexpr int NT = Gh::NT, FP = Gh::FP;
     constexpr int STRIDE = 1 << ((LG - 1) / 2), SUB = 2 * STRIDE, NSTEP = F / SUB, NFSTEP = NCH / STRIDE;
@@ -655,6 +669,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
     const int ant = (lid >> 1) - grp * p.nant;
     const int fgo = args.fg0 + grp;
     const int sig = 2 * ant + pol;
+    const int kq = fgo / args.kc;
+    const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
     int tid = threadIdx.x;
 
     for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
@@ -679,7 +695,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
 #pragma unroll
                 for (int r = 0; r < RL; r++) {
                     const int c = tid + m * NT + r * PL;
-                    store_slot<PREC>(args, ((long long)c * args.nfg + fgo) * p.Spad + sig, gi,
+                    store_slot<PREC>(args, xrow0 + (long long)c * args.kc * p.Spad, gi,
                                      make_float2(0.0f, 0.0f), 0.0f);
                     if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = make_float2(0.0f, 0.0f);
                 }
@@ -750,8 +766,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
                 for (int r = 0; r < R0; r++) wr[r] = x[r];
             }
             __syncthreads();
-            mid_pass<LG - 1, R1, R0, true, 2>(buf, tw1, args.tw, tid);
-            mid_pass<LG - 1, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
+            mid_pass<Gh, R1, R0, true, 2>(buf, tw1, args.tw, tid);
+            mid_pass<Gh, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
             // ---- last pass of half h, every bin
 #pragma unroll
             for (int m = 0; m < PERL; m++) {
@@ -777,7 +793,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
                         const float2 y = cmul(fr, X);
                         float si = 1.0f;
                         if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + sig];
-                        store_slot<PREC>(args, ((long long)c * args.nfg + fgo) * p.Spad + sig, gi, y, si);
+                        store_slot<PREC>(args, xrow0 + (long long)c * args.kc * p.Spad, gi, y, si);
                         if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
                     }
                 }
@@ -787,23 +803,25 @@ void fx_fstage_eo_kernel(FStageArgs args)
     }
 }
 
-template <int PREC>
-static void launch_eo_prec(const FStageArgs& a, dim3 grid, dim3 block, bool tap, hipStream_t s)
+template <int V, int PREC>
+static void launch_eo_prec(const FStageArgs& a, dim3 grid, bool tap, hipStream_t s)
 {
-    if (tap) fx_fstage_eo_kernel<PREC, true><<<grid, block, 0, s>>>(a);
-    else fx_fstage_eo_kernel<PREC, false><<<grid, block, 0, s>>>(a);
+    const dim3 block(Geo<kEoLG - 1, V>::NT);
+    if (tap) fx_fstage_eo_kernel<V, PREC, true><<<grid, block, 0, s>>>(a);
+    else fx_fstage_eo_kernel<V, PREC, false><<<grid, block, 0, s>>>(a);
 }
 
+template <int V>
 static hipError_t launch_eo(const FStageArgs& a, int prec, hipStream_t s)
 {
     const int nblocks = 2 * a.ngroups * a.p.nant;
     if (nblocks <= 0) return hipSuccess;
-    const dim3 grid(nblocks), block(Geo<kEoLG - 1>::NT);
+    const dim3 grid(nblocks);
     const bool tap = a.tap != nullptr;
     switch (prec) {
-    case FX_PREC_F32: launch_eo_prec<FX_PREC_F32>(a, grid, block, tap, s); break;
-    case FX_PREC_F16: launch_eo_prec<FX_PREC_F16>(a, grid, block, tap, s); break;
-    case FX_PREC_F8: launch_eo_prec<FX_PREC_F8>(a, grid, block, tap, s); break;
+    case FX_PREC_F32: launch_eo_prec<V, FX_PREC_F32>(a, grid, tap, s); break;
+    case FX_PREC_F16: launch_eo_prec<V, FX_PREC_F16>(a, grid, tap, s); break;
+    case FX_PREC_F8: launch_eo_prec<V, FX_PREC_F8>(a, grid, tap, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -839,8 +857,8 @@ __global__ __launch_bounds__(Geo<kEoLG - 1>::NT) void fx_fft_c2c_eo_kernel(const
             for (int r = 0; r < R0; r++) buf[pad(tid * R0 + r)] = x[r];
         }
         __syncthreads();
-        mid_pass<kEoLG - 1, R1, R0, true, 2>(buf, tw1, tw, tid);
-        mid_pass<kEoLG - 1, R2, R0 * R1, false, 2>(buf, tw1, tw, tid);
+        mid_pass<Gh, R1, R0, true, 2>(buf, tw1, tw, tid);
+        mid_pass<Gh, R2, R0 * R1, false, 2>(buf, tw1, tw, tid);
 #pragma unroll
         for (int m = 0; m < PERL; m++) {
             const int i = tid + m * NT;
@@ -898,8 +916,8 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* _
         for (int r = 0; r < R0; r++) buf[pad(i * R0 + r)] = x[r];
     }
     __syncthreads();
-    mid_pass<LG, R1, R0, true>(buf, tw1, tw, tid);
-    if constexpr (Gm::NP == 4) mid_pass<LG, R2, R0 * R1, false>(buf, tw1, tw, tid);
+    mid_pass<Gm, R1, R0, true>(buf, tw1, tw, tid);
+    if constexpr (Gm::NP == 4) mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, tw, tid);
 #pragma unroll
     for (int m = 0; m < PERL; m++) {
         const int i = tid + m * NT;
@@ -940,33 +958,44 @@ hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int l
 
 bool fstage_supported(int lg) { return lg >= 9 && lg <= 15; }
 
-template <int LG>
-static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
+template <int LG, int V>
+static hipError_t launch_lgv(const FStageArgs& a, int prec, hipStream_t s)
 {
     const int nblocks = 2 * a.ngroups * a.p.nant;     // one block per polarisation
     if (nblocks <= 0) return hipSuccess;
-    const dim3 grid(nblocks), block(Geo<LG>::NT);
+    const dim3 grid(nblocks), block(Geo<LG, V>::NT);
     const bool tap = a.tap != nullptr;
     switch (prec) {
     case FX_PREC_F32:
-        if (tap) fx_fstage_kernel<LG, FX_PREC_F32, true><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, FX_PREC_F32, false><<<grid, block, 0, s>>>(a);
+        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F32, true><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, V, FX_PREC_F32, false><<<grid, block, 0, s>>>(a);
         break;
     case FX_PREC_F16:
-        if (tap) fx_fstage_kernel<LG, FX_PREC_F16, true><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, FX_PREC_F16, false><<<grid, block, 0, s>>>(a);
+        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F16, true><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, V, FX_PREC_F16, false><<<grid, block, 0, s>>>(a);
         break;
     case FX_PREC_F8:
-        if (tap) fx_fstage_kernel<LG, FX_PREC_F8, true><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, FX_PREC_F8, false><<<grid, block, 0, s>>>(a);
+        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F8, true><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, V, FX_PREC_F8, false><<<grid, block, 0, s>>>(a);
         break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <int LG>
+static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
+{
+    if constexpr (nplans(LG) > 1)
+        if (a.plan == 1) return launch_lgv<LG, 1>(a, prec, s);
+    return launch_lgv<LG, 0>(a, prec, s);
+}
+
+int fstage_plans(int lg) { return lg == kEoLG ? nplans(kEoLG - 1) : nplans(lg); }
+
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
 {
+    if (a.plan < 0 || a.plan >= fstage_plans(a.p.lg)) return hipErrorInvalidValue;
     switch (a.p.lg) {
     case 9:  return launch_lg<9>(a, prec, s);
     case 10: return launch_lg<10>(a, prec, s);
@@ -974,7 +1003,7 @@ hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
     case 12: return launch_lg<12>(a, prec, s);
     case 13: return launch_lg<13>(a, prec, s);
     case 14: return launch_lg<14>(a, prec, s);
-    case 15: return launch_eo(a, prec, s);
+    case 15: return a.plan == 1 ? launch_eo<1>(a, prec, s) : launch_eo<0>(a, prec, s);
     default: return hipErrorInvalidValue;
     }
 }

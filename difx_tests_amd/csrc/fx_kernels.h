@@ -14,19 +14,19 @@ struct FStageArgs {
     const float2* tw;             // [F] exp(-2 pi i m / F)
     const float* subchf;          // [stride] f32 channel frequencies (kernel.c:164)
     const float* stepchf;         // [nfstep] (kernel.c:165)
-    void* out;                    // X input, [nchan][nfg][Spad][G]
-    int nfg;                      // groups held by the X input buffer
+    void* out;                    // X input, [nfg/kc][nchan][kc][Spad] rows of G FFTs
+    int kc;                       // groups per X-engine K chunk
     int fg0;                      // first group written by this launch
     int ngroups;                  // groups in this launch
     const float* scale_inv;       // FP8: [nchan][Spad] inverse scales
     float2* tap;                  // optional FP32 spectra [nffts][nant][2][nchan]
+    int plan;                     // radix plan, 0 .. fstage_plans(lg) - 1
 };
 
 struct XArgs {
     FxPlan p;
-    const void* xin;              // [nchan][nfg][Spad][G]
-    int nfg;                      // groups held by the buffer (row stride)
-    int fg0, ngroups;             // group range reduced by this launch
+    const void* xin;              // [nfg/KC][nchan][KC][Spad] rows of G FFTs
+    int fg0, ngroups;             // group range reduced by this launch (multiples of KC)
     float2* vacc;                 // [nchan][nbp][32*32]
     int accumulate;               // 0: overwrite, 1: add
     const float* scale;           // FP8: [nchan][Spad] scales
@@ -39,6 +39,7 @@ hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fil
                         int numffts, int nffts_pad, int maxoffset, FxDelay* out, hipStream_t s);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
 bool fstage_supported(int lg);
+int fstage_plans(int lg);
 // Batched forward C2C FFT of length 2^lg (natural order in/out, unnormalised).
 hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int lg, int batch, hipStream_t s);
 

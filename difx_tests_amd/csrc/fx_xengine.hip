@@ -14,11 +14,12 @@
 // lane -> k assignment, so the sum is over every k whatever order the
 // instruction reduces them in.
 //
-// The F-stage wrote the corner-turned [chan][group][signal][G] layout, so a
-// 16-byte row holds exactly one lane's K slice of one signal; K chunks of
-// two signal panels are staged through LDS and each wave keeps the
-// accumulators of up to NBW blocks resident for the whole FFT range of the
-// launch.
+// The F-stage wrote the corner-turned [chunk][chan][group][signal] layout of
+// 16-byte rows (G FFTs of one signal), so a row holds exactly one lane's K
+// slice of one signal and a K chunk of one channel is KC*Spad contiguous
+// rows; K chunks of two signal panels are staged through LDS and each wave
+// keeps the accumulators of up to NBW blocks resident for the whole FFT range
+// of the launch.
 #include <hip/hip_runtime.h>
 
 #include "fx_common.h"
@@ -152,7 +153,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         }
     }
 
-    const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
+    // chunk q of channel c: KC*Spad contiguous rows at ((q*nchan + c)*KC)*Spad
+    const long long cstride = (long long)a.p.nchan * KC * Spad;
+    const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * KC * Spad;
     const int nchunk = a.ngroups / KC;
     // per-thread chunk loads: element e = (panel, group, signal), signal fastest
     uint4 st[NLD];
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             const int sig = (pan ? SJ : SI) * PW + sgl;
             st[l] = make_uint4(0, 0, 0, 0);
             if (e < NELEM && sig < Spad && !(diag && pan))
-                st[l] = gbase[(long long)(chunk * KC + g) * Spad + sig];
+                st[l] = gbase[chunk * cstride + g * Spad + sig];
         }
     };
     auto sstore = [&](int stage) {
@@ -303,7 +306,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const bool diag = SI == SJ;
     const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
     const int pb = diag ? 0 : 1;
-    const float4* gbase = reinterpret_cast<const float4*>(a.xin) + ((long long)c * a.nfg + a.fg0) * Spad;
+    const long long cstride = (long long)a.p.nchan * KC * Spad;
+    const float4* gbase =
+        reinterpret_cast<const float4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * KC * Spad;
     const int nchunk = a.ngroups / KC;
     const int n = lane & 31, kg = lane >> 5;
 
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
                 const int g = rem / PW, sgl = rem - g * PW;
                 const int sig = (pan ? SJ : SI) * PW + sgl;
                 st[l] = make_float4(0, 0, 0, 0);
-                if (e < NELEM && sig < Spad && !(diag && pan)) st[l] = gbase[(long long)(chunk * KC + g) * Spad + sig];
+                if (e < NELEM && sig < Spad && !(diag && pan)) st[l] = gbase[chunk * cstride + g * Spad + sig];
             }
         };
         auto sstore = [&](int stage) {

@@ -24,8 +24,9 @@ NATIVE_SYMBOLS = [
     "fx_read_vis", "fx_read_delay_table", "fx_enable_tap", "fx_read_tap", "fx_set_fp8_scales", "fx_sync",
     "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry", "fx_set_option", "fx_get_option",
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
+    "fx_autotune",
 ]
-OPT_XTILE = 1
+OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH = 1, 2, 3, 4
 BATCH_LAUNCH, BATCH_GRAPH = 0, 1
 # Every symbol include/kernel.h declares (kernel.h:61-134 of the reference).
 KERNELH_SYMBOLS = [
@@ -93,6 +94,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_batch_process.argtypes = [P, P, P, I, I]
     lib.fx_batch_sync.argtypes = [P]
     lib.fx_batch_captures.argtypes = [P]
+    lib.fx_autotune.argtypes = [P, I, I, C.c_char_p, P]
     _lib = lib
     return lib
 
@@ -201,14 +203,39 @@ class FxEngine:
         self._check(self.lib.fx_set_fp8_scales(self.h, _ptr(s)))
 
     # ---- tuning
+    def _get(self, opt: int) -> int:
+        v = C.c_int()
+        self._check(self.lib.fx_get_option(self.h, opt, C.byref(v)))
+        return v.value
+
     def set_xtile(self, sb: int) -> None:
         """X-engine workgroup tile (1, 2 or 4 blocks per panel side; 0: automatic)."""
         self._check(self.lib.fx_set_option(self.h, OPT_XTILE, sb))
 
     def xtile(self) -> int:
-        v = C.c_int()
-        self._check(self.lib.fx_get_option(self.h, OPT_XTILE, C.byref(v)))
-        return v.value
+        return self._get(OPT_XTILE)
+
+    def set_plan(self, v: int) -> None:
+        """F-stage radix plan, 0 .. nplans() - 1."""
+        self._check(self.lib.fx_set_option(self.h, OPT_FPLAN, v))
+
+    def plan(self) -> int:
+        return self._get(OPT_FPLAN)
+
+    def nplans(self) -> int:
+        return self._get(OPT_NPLANS)
+
+    def xbatch(self) -> int:
+        """Subints per X-engine launch recommended by the autotuner."""
+        return self._get(OPT_XBATCH)
+
+    def autotune(self, numffts: int, reps: int = 3, cache: str | None = None) -> dict:
+        """Times the radix plans, X-engine tiles and subints per X launch on the
+        uploaded input; keeps the fastest and zeroes the visibilities."""
+        rep = np.zeros(5, np.float64)
+        self._check(self.lib.fx_autotune(self.h, numffts, reps, cache.encode() if cache else None, _ptr(rep)))
+        return {"plan": int(rep[0]), "xtile": int(rep[1]), "xbatch": int(rep[2]), "fstage_ms": float(rep[3]),
+                "xstage_ms_per_subint": float(rep[4])}
 
     # ---- timing
     def set_timing(self, on: bool = True) -> None:

@@ -120,11 +120,32 @@ int fx_get_timing(fx_context *ctx, double *ms4, int *nlaunch2);
 int fx_get_geometry(const fx_context *ctx, int *out11);
 
 /* Tuning options (the autotuner sets these; results do not depend on them).
- * FX_OPT_XTILE: X-engine workgroup tile, a square of 1, 2 or 4 blocks of
- * 32x32 signals per panel; 0 selects by the signal count. */
+ * FX_OPT_XTILE   X-engine workgroup tile, a square of 1, 2 or 4 blocks of
+ *                32x32 signals per panel; 0 selects by the signal count
+ * FX_OPT_FPLAN   F-stage radix plan, 0 .. FX_OPT_NPLANS - 1
+ * FX_OPT_NPLANS  (read only) number of radix plans at this FFT length
+ * FX_OPT_XBATCH  subints per X-engine launch recommended to callers that
+ *                batch fx_fstage calls before one fx_xstage (default 1) */
 #define FX_OPT_XTILE 1
+#define FX_OPT_FPLAN 2
+#define FX_OPT_NPLANS 3
+#define FX_OPT_XBATCH 4
 int fx_set_option(fx_context *ctx, int option, int value);
 int fx_get_option(const fx_context *ctx, int option, int *value);
+
+/* Autotuner: times every F-stage radix plan, every X-engine tile and 1, 2,
+ * 4, ... subints per X-engine launch (up to the FFT capacity) on this
+ * context's uploaded input and delay model, numffts FFTs per subint, each
+ * candidate once untimed then `reps` times; keeps the fastest of each (for the
+ * launch count, timed as whole steps of that many F-stages and one X-engine
+ * launch, the smallest within 3% of the best time per subint) and zeroes the
+ * visibilities and counts.  report (may be NULL) receives {plan, tile,
+ * subints per launch, F-stage ms per subint, X-engine ms per subint over one
+ * subint}.  When
+ * cache_path (or, if it is NULL, the FX_TUNE_CACHE environment variable) names
+ * a file, the result is stored there per (GPU arch, nant, nchan, prec), and
+ * fx_create applies a stored entry when FX_TUNE_CACHE names that file. */
+int fx_autotune(fx_context *ctx, int numffts, int reps, const char *cache_path, double *report);
 
 /* Batches: several contexts (one per subband, each with its own antennas,
  * channels, bandwidth and precision) processed one subint at a time.

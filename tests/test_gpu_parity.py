@@ -79,11 +79,15 @@ def test_delay_table_bit_exact(nchan):
     assert 0 < ref["valid"].sum() < nf * nant          # the case exercises invalid antennas
 
 
+@pytest.mark.parametrize("plan", [0, 1])
 @pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096, 8192, 16384])
-def test_fstage_fp32_matches_oracle(nchan):
+def test_fstage_fp32_matches_oracle(nchan, plan):
     nant, nf = 3, 6
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 20 + nchan)
     with _engine(p, nant, nf, data=data) as e:
+        if plan >= e.nplans():
+            pytest.skip(f"one radix plan at nchan={nchan}")
+        e.set_plan(plan)
         e.set_model(delays, fileoff)
         e.upload(data)
         e.enable_tap(True)
@@ -207,3 +211,51 @@ def test_vis_fp8_vs_fp64_oracle(nchan, nant, nf):
     ex = _rel(autos[:, [1, 2]], ref["autos"][:, [1, 2]]).max()
     print(f"fp8 max rel error: cross {ec:.3e} parallel-hand auto {ea:.3e} cross-hand auto {ex:.3e}")
     assert ec <= 0.1 and ex <= 0.1 and ea <= 0.02
+
+
+@pytest.mark.parametrize("prec", ["f16", "f8"])
+@pytest.mark.parametrize("nchan", [1024, 4096, 16384])
+def test_radix_plans_agree_in_low_precision(nchan, prec):
+    # both plans feed the FP16 / FP8 X-engine: same counts, visibilities
+    # within the FP32 spectrum difference
+    nant, nf = 4, 16
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 60 + nchan, data="correlated")
+    out = []
+    for plan in (0, 1):
+        with _engine(p, nant, nf, prec=prec, data=data) as e:
+            e.set_plan(plan)
+            e.set_model(delays, fileoff)
+            e.upload(data)
+            e.reset()
+            e.process(it)
+            out.append(e.read_vis())
+    assert np.array_equal(out[0][1], out[1][1])
+    live = out[0][1] > 0
+    tol = 1e-3 if prec == "f16" else 0.05
+    assert _rel(out[1][0][live], out[0][0][live]).max() <= tol
+
+
+def test_autotune_and_cache(tmp_path, monkeypatch):
+    nchan, nant, nf = 2048, 6, 12
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 71)
+    cache = str(tmp_path / "tune.txt")
+    with _engine(p, nant, 4 * nf, prec="f32", data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        rep = e.autotune(nf, reps=2, cache=cache)
+        assert rep["plan"] in range(e.nplans()) and rep["xtile"] in (1, 2, 4) and rep["xbatch"] in (1, 2, 4)
+        assert e.plan() == rep["plan"] and e.xtile() == rep["xtile"] and e.xbatch() == rep["xbatch"]
+        cross, cc, autos, ac = e.read_vis()              # the tuner leaves the visibilities zeroed
+        assert not np.any(cc) and not np.any(cross) and not np.any(autos)
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f32(p, data, delays, fileoff, it)
+    assert np.array_equal(cc, ref["count"])
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= 1e-5
+    lines = [l.split() for l in open(cache) if not l.startswith("#")]
+    assert len(lines) == 1 and lines[0][1:4] == [str(nant), str(nchan), "0"]
+    # a new context with FX_TUNE_CACHE set starts from the stored choice
+    monkeypatch.setenv("FX_TUNE_CACHE", cache)
+    with _engine(p, nant, nf, prec="f32", data=data) as e2:
+        assert (e2.plan(), e2.xtile(), e2.xbatch()) == (rep["plan"], rep["xtile"], rep["xbatch"])
