@@ -47,17 +47,36 @@ def parse():
     ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8", "f32v"])
     ap.add_argument("--xbatch", type=int, default=0, help="subints per X-engine launch (0: the autotuner's choice)")
     ap.add_argument("--no-autotune", action="store_true", help="default radix plan and X-engine tile, xbatch 8")
+    ap.add_argument("--plan", type=int, default=None, help="F-stage radix plan (skips the autotuner)")
+    ap.add_argument("--xtile", type=int, default=None, help="X-engine tile 1, 2 or 4 (skips the autotuner)")
     ap.add_argument("--no-fp32-compare", action="store_true")
     ap.add_argument("--no-subbands", action="store_true", help="skip the eight-subband batch measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work (s); 0 disables")
     ap.add_argument("--seed", type=int, default=20261205)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three plain 512 MiB device copies (calibrates the HBM traffic counters)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """Three device-to-device copies of 512 MiB: FETCH_SIZE / WRITE_SIZE of
+    this kernel against the bytes it must move give the counter scale."""
+    import torch
+    n = 512 * 1024 * 1024
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
 
 
 MAX_XBATCH = 16
 
 
-def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune=True):
+def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune=True, plan=None, xtile=None):
     """Returns (seconds per step, kernel ms per step dict, launch counts, slots
     per subint, tuning dict).  xbatch 0 takes the autotuner's choice."""
     nf = len(it)
@@ -67,6 +86,11 @@ def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune
         e.set_model(delays, fileoff)
         e.upload(data)
         tuned = e.autotune(nf, reps=3) if tune else {}
+        if plan is not None:
+            e.set_plan(plan)
+        if xtile is not None:
+            e.set_xtile(xtile)
+        tuned["plan_used"], tuned["xtile_used"] = e.plan(), e.xtile()
         if not xbatch:
             xbatch = tuned.get("xbatch", 8)
         tuned["xbatch_used"] = xbatch
@@ -198,6 +222,9 @@ def cpu_baseline(w, data, delays, fileoff, budget_s):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not implemented)"}))
@@ -206,9 +233,10 @@ def main():
     prec = a.prec or w.prec
     data, delays, fileoff, it = workloads.make_subint(w, a.seed)
     nf = len(it)
-    tune = not a.no_autotune
+    tune = not (a.no_autotune or a.plan is not None or a.xtile is not None)
     xb_req = a.xbatch or (0 if tune else 8)
-    sec, kms, nl, slots, tuned = run_pipeline(w, prec, data, delays, fileoff, it, a.steps, a.warmup, xb_req, tune)
+    sec, kms, nl, slots, tuned = run_pipeline(w, prec, data, delays, fileoff, it, a.steps, a.warmup, xb_req, tune,
+                                              a.plan, a.xtile)
     xbatch = tuned["xbatch_used"]
     samples_per_step = xbatch * w.samples_per_subint
     value = samples_per_step / sec / 1e9
@@ -254,7 +282,7 @@ def main():
         "fstage_gbs": round(f_gbs, 1),
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kms.items()},
         "realtime_factor": round((xbatch * w.subint_s) / sec, 2),
-        "autotune": tuned if tune else None,
+        "tuning": tuned,
         "roofline": roof,
     }
     if not a.no_fp32_compare and prec not in ("f32", "f32v"):
@@ -265,7 +293,7 @@ def main():
             v32 = samples_per_step / s32 / 1e9
             out[key] = {"xengine": p32, "value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
                         "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
-                        "autotune": t32 if tune else None, "speedup_vs_this": round(value / v32, 3)}
+                        "tuning": t32, "speedup_vs_this": round(value / v32, 3)}
     if not a.no_subbands:
         out["subband_batch"] = subband_batch(max(5, a.steps), 2, a.seed, tune=tune)
     if a.cpu_seconds > 0:
