@@ -526,76 +526,107 @@ int or_process_f64(const or_plan *p, int nant, int numffts, int split,
                    double *vis, int *count, double *autos, int *autocount,
                    double *chan, int *valid_out)
 {
+    /* OpenMP over antennas (F part) and over the first antenna of each
+     * baseline row (X part); every accumulator is written by one thread in
+     * the same order as the serial loop, so the result does not depend on
+     * the thread count. */
     const int F = p->F, nchan = p->nchan;
     const int maxoffset = (numffts * split - 1) * F;
     fft_prepare_all(F);
-    float *l0 = (float *)malloc(sizeof(float) * (size_t)F);
-    float *l1 = (float *)malloc(sizeof(float) * (size_t)F);
-    or_cf64 *rot = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)F);
-    or_cf64 *u = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)F);
-    or_cf64 *y = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)F);
-    or_cf64 *fr = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)nchan);
+#ifdef _OPENMP
+    const int nt = omp_get_max_threads();
+#else
+    const int nt = 1;
+#endif
+    float *l0 = (float *)malloc(sizeof(float) * (size_t)F * nt);
+    float *l1 = (float *)malloc(sizeof(float) * (size_t)F * nt);
+    or_cf64 *rot = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)F * nt);
+    or_cf64 *u = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)F * nt);
+    or_cf64 *y = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)F * nt);
+    or_cf64 *fr = (or_cf64 *)malloc(sizeof(or_cf64) * (size_t)nchan * nt);
     or_cf64 *spec = (or_cf64 *)calloc((size_t)nant * 2 * nchan, sizeof(or_cf64));
     int *valid = (int *)malloc(sizeof(int) * (size_t)nant);
-    if (!l0 || !l1 || !rot || !u || !y || !fr || !spec || !valid) return -1;
-    for (int k = 0; k < numffts; k++) {
-        for (int j = 0; j < nant; j++) {
-            double mean, a, b, fd;
-            int off;
-            or_station_delay(delays + 3 * j, iteration[k], &mean, &a, &b);
-            valid[j] = or_offset(mean, fileoffsets[j], p->Ts, iteration[k], F, maxoffset, &off, &fd);
-            if (valid_out) valid_out[(size_t)k * nant + j] = valid[j];
-            or_cf64 *sj = spec + (size_t)j * 2 * nchan;
-            if (valid[j]) {
-                or_unpack2bit(data[j], off, F, l0, l1);
-                or_fringe_rotator_f64(p, a, b, rot);
-                or_frac_rotator_f64(p, fd, fr);
-                for (int pol = 0; pol < 2; pol++) {
-                    const float *l = pol ? l1 : l0;
-                    for (int i = 0; i < F; i++) {
-                        u[i].re = (double)l[i] * rot[i].re;
-                        u[i].im = (double)l[i] * rot[i].im;
+    if (!l0 || !l1 || !rot || !u || !y || !fr || !spec || !valid) {
+        free(l0); free(l1); free(rot); free(u); free(y); free(fr); free(spec); free(valid);
+        return -1;
+    }
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+#ifdef _OPENMP
+        const int tn = omp_get_thread_num();
+#else
+        const int tn = 0;
+#endif
+        float *tl0 = l0 + (size_t)tn * F, *tl1 = l1 + (size_t)tn * F;
+        or_cf64 *trot = rot + (size_t)tn * F, *tu = u + (size_t)tn * F, *ty = y + (size_t)tn * F;
+        or_cf64 *tfr = fr + (size_t)tn * nchan;
+        for (int k = 0; k < numffts; k++) {
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+            for (int j = 0; j < nant; j++) {
+                double mean, a, b, fd;
+                int off;
+                or_station_delay(delays + 3 * j, iteration[k], &mean, &a, &b);
+                valid[j] = or_offset(mean, fileoffsets[j], p->Ts, iteration[k], F, maxoffset, &off, &fd);
+                if (valid_out) valid_out[(size_t)k * nant + j] = valid[j];
+                or_cf64 *sj = spec + (size_t)j * 2 * nchan;
+                if (valid[j]) {
+                    or_unpack2bit(data[j], off, F, tl0, tl1);
+                    or_fringe_rotator_f64(p, a, b, trot);
+                    or_frac_rotator_f64(p, fd, tfr);
+                    for (int pol = 0; pol < 2; pol++) {
+                        const float *l = pol ? tl1 : tl0;
+                        for (int i = 0; i < F; i++) {
+                            tu[i].re = (double)l[i] * trot[i].re;
+                            tu[i].im = (double)l[i] * trot[i].im;
+                        }
+                        or_fft_f64(F, tu, ty);
+                        for (int ch = 0; ch < nchan; ch++) sj[pol * nchan + ch] = cmul64(tfr[ch], ty[ch]);
                     }
-                    or_fft_f64(F, u, y);
-                    for (int ch = 0; ch < nchan; ch++) sj[pol * nchan + ch] = cmul64(fr[ch], y[ch]);
+                }
+                if (chan) {
+                    double *dst = chan + ((size_t)k * nant + j) * 2 * nchan * 2;
+                    if (valid[j])
+                        memcpy(dst, sj, sizeof(or_cf64) * 2 * (size_t)nchan);
+                    else
+                        memset(dst, 0, sizeof(or_cf64) * 2 * (size_t)nchan);
                 }
             }
-            if (chan) {
-                double *dst = chan + ((size_t)k * nant + j) * 2 * nchan * 2;
-                if (valid[j])
-                    memcpy(dst, sj, sizeof(or_cf64) * 2 * (size_t)nchan);
-                else
-                    memset(dst, 0, sizeof(or_cf64) * 2 * (size_t)nchan);
-            }
-        }
-        int b = 0;
-        for (int j = 0; j < nant; j++) {
-            for (int kk = j + 1; kk < nant; kk++, b++) {
-                if (!valid[j] || !valid[kk]) continue;
-                for (int l = 0; l < 2; l++)
-                    for (int m = 0; m < 2; m++) {
-                        const or_cf64 *x = spec + ((size_t)j * 2 + l) * nchan;
-                        const or_cf64 *yy = spec + ((size_t)kk * 2 + m) * nchan;
-                        double *acc = vis + (((size_t)b * 4 + 2 * l + m) * nchan) * 2;
-                        for (int ch = 0; ch < nchan; ch++) {
-                            acc[2 * ch] += x[ch].re * yy[ch].re + x[ch].im * yy[ch].im;
-                            acc[2 * ch + 1] += x[ch].im * yy[ch].re - x[ch].re * yy[ch].im;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+            for (int j = 0; j < nant; j++) {
+                int b = j * nant - j * (j + 1) / 2;             /* first baseline (j, j+1), j-major */
+                for (int kk = j + 1; kk < nant; kk++, b++) {
+                    if (!valid[j] || !valid[kk]) continue;
+                    for (int l = 0; l < 2; l++)
+                        for (int m = 0; m < 2; m++) {
+                            const or_cf64 *x = spec + ((size_t)j * 2 + l) * nchan;
+                            const or_cf64 *yy = spec + ((size_t)kk * 2 + m) * nchan;
+                            double *acc = vis + (((size_t)b * 4 + 2 * l + m) * nchan) * 2;
+                            for (int ch = 0; ch < nchan; ch++) {
+                                acc[2 * ch] += x[ch].re * yy[ch].re + x[ch].im * yy[ch].im;
+                                acc[2 * ch + 1] += x[ch].im * yy[ch].re - x[ch].re * yy[ch].im;
+                            }
                         }
-                    }
-                count[b]++;
-            }
-            if (autos && valid[j]) {
-                for (int l = 0; l < 2; l++)
-                    for (int m = 0; m < 2; m++) {
-                        const or_cf64 *x = spec + ((size_t)j * 2 + l) * nchan;
-                        const or_cf64 *yy = spec + ((size_t)j * 2 + m) * nchan;
-                        double *acc = autos + (((size_t)j * 4 + 2 * l + m) * nchan) * 2;
-                        for (int ch = 0; ch < nchan; ch++) {
-                            acc[2 * ch] += x[ch].re * yy[ch].re + x[ch].im * yy[ch].im;
-                            acc[2 * ch + 1] += x[ch].im * yy[ch].re - x[ch].re * yy[ch].im;
+                    count[b]++;
+                }
+                if (autos && valid[j]) {
+                    for (int l = 0; l < 2; l++)
+                        for (int m = 0; m < 2; m++) {
+                            const or_cf64 *x = spec + ((size_t)j * 2 + l) * nchan;
+                            const or_cf64 *yy = spec + ((size_t)j * 2 + m) * nchan;
+                            double *acc = autos + (((size_t)j * 4 + 2 * l + m) * nchan) * 2;
+                            for (int ch = 0; ch < nchan; ch++) {
+                                acc[2 * ch] += x[ch].re * yy[ch].re + x[ch].im * yy[ch].im;
+                                acc[2 * ch + 1] += x[ch].im * yy[ch].re - x[ch].re * yy[ch].im;
+                            }
                         }
-                    }
-                if (autocount) autocount[j]++;
+                    if (autocount) autocount[j]++;
+                }
             }
         }
     }
