@@ -57,6 +57,7 @@ struct fx_context {
     double* d_model;    // delays [nant*3] then fileoffsets [nant]
     int* d_iter;
     FxDelay* d_dtab;
+    float2* d_rtab;     // rotator tables [max_ffts][nant][sub + nstep + stride + nfstep]
     float2* d_tw;
     float* d_subchf;
     float* d_stepchf;
@@ -65,8 +66,8 @@ struct fx_context {
     float2* d_vacc;
     size_t vacc_bytes;
     int* d_counts;
-    float* d_scale;     // [nchan][Spad] (FP8)
-    float* d_scale_inv;
+    float* d_scale;     // [nchan][Spad] (FP8, X-engine)
+    float* d_scale_inv; // [Spad][nchan] (FP8, F-stage)
     float2* d_cross;
     float2* d_autos;
     float2* d_tap;
@@ -95,7 +96,7 @@ extern "C" int fx_fft_granule(const fx_context* ctx) { return ctx ? ctx->granule
 
 static void free_ctx(fx_context* c)
 {
-    void* bufs[] = {c->d_data, c->d_model, c->d_iter, c->d_dtab, c->d_tw, c->d_subchf, c->d_stepchf,
+    void* bufs[] = {c->d_data, c->d_model, c->d_iter, c->d_dtab, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
                     c->d_xin, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv,
                     c->d_cross, c->d_autos, c->d_tap};
     for (void* b : bufs)
@@ -194,6 +195,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     ALLOC(c->d_model, sizeof(double) * 4 * p.nant);
     ALLOC(c->d_iter, sizeof(int) * c->max_ffts);
     ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
+    ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * (p.sub + p.nstep + p.stride + p.nfstep));
     ALLOC(c->d_tw, sizeof(float2) * p.F);
     ALLOC(c->d_subchf, sizeof(float) * p.stride);
     ALLOC(c->d_stepchf, sizeof(float) * p.nfstep);
@@ -380,11 +382,13 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     const FxPlan& p = c->p;
     TIMED(TM_DELAY, s, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, f.numffts, f.npad,
                                     f.maxoffset, c->d_dtab, s));
+    TIMED(TM_DELAY, s, launch_rotators(p, c->d_dtab, f.npad, c->d_subchf, c->d_stepchf, c->d_rtab, s));
     FStageArgs a;
     a.p = p;
     a.data = c->d_data;
     a.data_stride = c->stride;
     a.delays = c->d_dtab;
+    a.rtab = c->d_rtab;
     a.tw = c->d_tw;
     a.subchf = c->d_subchf;
     a.stepchf = c->d_stepchf;
@@ -526,7 +530,7 @@ extern "C" int fx_set_fp8_scales(fx_context* c, const float* scales)
             const float v = scales[(size_t)ch * p.S + s];
             if (!(v > 0.0f) || !std::isfinite(v)) return fail(FX_EINVAL, "scale[%d][%d] not positive", ch, s);
             sc[(size_t)ch * p.Spad + s] = v;
-            sci[(size_t)ch * p.Spad + s] = 1.0f / v;
+            sci[(size_t)s * p.nchan + ch] = 1.0f / v;
         }
     HIPCHK(hipMemcpyAsync(c->d_scale, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_scale_inv, sci.data(), sizeof(float) * sci.size(), hipMemcpyHostToDevice, c->stream));

@@ -161,6 +161,63 @@ __device__ __forceinline__ float2 sincos_rounded(float xf)
     return make_float2((float)co, (float)so);
 }
 
+// --------------------------------------------------------------------------
+// Rotator tables per (FFT, antenna), shared by both polarisations:
+//   [0, sub)                      fringe rotator of sample s within a block
+//   [sub, sub + nstep)            fringe rotator of block k (k = 0 is exactly 1)
+//   [.., + stride)                fractional-delay rotator of channel s
+//   [.., + nfstep)                fractional-delay rotator of channel block k
+// FP64 phase arguments in the reference's operation order (kernel.c:428-497,
+// 512-555; contraction off), sin/cos rounded to FP32.
+
+__global__ void fx_rotator_kernel(FxPlan p, const FxDelay* __restrict__ dtab, const float* __restrict__ subchf,
+                                  const float* __restrict__ stepchf, float2* __restrict__ rtab)
+{
+#pragma clang fp contract(off)
+    const int ntab = p.sub + p.nstep + p.stride + p.nfstep;
+    const long long fa = blockIdx.x;                  // (FFT, antenna)
+    const FxDelay d = dtab[fa];
+    if (!d.valid) return;
+    float2* out = rtab + fa * ntab;
+    const int integerdelay = (int)d.b;
+    const double bfrac = d.b - integerdelay;
+    const double lofr = p.lofrac * (double)integerdelay;
+    const float fd32 = (float)d.fd;
+    for (int e = threadIdx.x; e < ntab; e += blockDim.x) {
+        float arg;
+        bool unit = false;
+        if (e < p.sub) {
+            double subxval = ((double)e / (double)p.F) * d.a;
+            subxval += bfrac;
+            double ph = subxval * p.lo;
+            if (p.fracLO) ph += lofr;
+            arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
+        } else if (e < p.sub + p.nstep) {
+            const int k = e - p.sub;
+            const double stepxval = (((double)k * (double)p.stride) / (double)p.F) * d.a;
+            const double ph = stepxval * p.lo;
+            arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
+            unit = (k == 0);
+        } else if (e < p.sub + p.nstep + p.stride) {
+            arg = subchf[e - p.sub - p.nstep] * fd32;
+        } else {
+            const int k = e - p.sub - p.nstep - p.stride;
+            arg = stepchf[k] * fd32;
+            unit = (k == 0);
+        }
+        out[e] = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
+    }
+}
+
+hipError_t launch_rotators(const FxPlan& p, const FxDelay* dtab, int nffts, const float* subchf,
+                           const float* stepchf, float2* rtab, hipStream_t s)
+{
+    const long long n = (long long)nffts * p.nant;
+    if (n <= 0) return hipSuccess;
+    fx_rotator_kernel<<<(unsigned)n, 128, 0, s>>>(p, dtab, subchf, stepchf, rtab);
+    return hipGetLastError();
+}
+
 __host__ __device__ constexpr int cbrev(int i, int bits)
 {
     int r = 0;
@@ -416,8 +473,14 @@ void fx_fstage_kernel(FStageArgs args)
     __shared__ __align__(16) float2 buf[FP];
     __shared__ __align__(16) float lut[2][256];         // [time slot][byte] -> level of this pol
     __shared__ __align__(16) uint32_t bytes[F / 8 + 4];
-    __shared__ float2 rsub[SUB], rstep[NSTEP], frsub[STRIDE], frstep[NFSTEP];
+    __shared__ __align__(16) float2 tab[Gm::NTAB];       // rotator tables of this (FFT, antenna)
     __shared__ float2 tw1[NTW1];                         // exp(-2 pi i m / (R0 R1))
+    const float2* rsub = tab;
+    const float2* rstep = tab + SUB;
+    const float2* frsub = tab + SUB + NSTEP;
+    const float2* frstep = tab + SUB + NSTEP + STRIDE;
+    constexpr int TPT = (Gm::NTAB + NT - 1) / NT;       // table entries per thread
+    constexpr int WPT = (F / 8 + 1 + NT - 1) / NT;       // window words per thread
 
     const FxPlan& p = args.p;
 
@@ -450,6 +513,36 @@ void fx_fstage_kernel(FStageArgs args)
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
     OutRegs<PREC, G, NOUT> outr;
 
+    // The next FFT's offset, validity, rotator tables and packed-byte window
+    // are loaded into registers while the current FFT runs.
+    struct Prefetch {
+        int valid, offset;
+        float2 t[TPT];
+        uint32_t w[WPT];
+    };
+    auto fetch = [&](int gi, Prefetch& f) {
+        const long long fa = (long long)(grp * G + gi) * p.nant + ant;
+        f.valid = args.delays[fa].valid;
+        f.offset = args.delays[fa].offset;
+        if (!f.valid) return;
+        const float2* tb = args.rtab + fa * Gm::NTAB;
+#pragma unroll
+        for (int t = 0; t < TPT; t++) {
+            const int e = threadIdx.x + t * NT;
+            f.t[t] = e < Gm::NTAB ? tb[e] : make_float2(0.0f, 0.0f);
+        }
+        const int bb = (f.offset >> 1) & ~3;
+        const int nd = ((((f.offset + F - 1) >> 1) - bb) >> 2) + 1;
+        const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src + bb);
+#pragma unroll
+        for (int w = 0; w < WPT; w++) {
+            const int k = threadIdx.x + w * NT;
+            f.w[w] = k < nd ? s32[k] : 0u;
+        }
+    };
+    Prefetch cur;
+    fetch(0, cur);
+
 #pragma unroll 1
     for (int gi = 0; gi < G; gi++) {
         // Redefine tid for the compiler on every FFT: the LDS addresses derived
@@ -459,9 +552,8 @@ void fx_fstage_kernel(FStageArgs args)
         __builtin_assume(tid >= 0 && tid < NT);
         outr.shift();
         const int fft = grp * G + gi;             // FFT of this launch
-        const FxDelay d = args.delays[(long long)fft * p.nant + ant];
         __syncthreads();                          // previous FFT done with LDS
-        if (!d.valid) {                           // uniform across the block
+        if (!cur.valid) {                         // uniform across the block
             outr.zero_last();
             if (TAP) {
 #pragma unroll
@@ -471,53 +563,24 @@ void fx_fstage_kernel(FStageArgs args)
                         args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + tid + m * NT + r * PL] =
                             make_float2(0.0f, 0.0f);
             }
+            if (gi + 1 < G) fetch(gi + 1, cur);
             continue;
         }
-        // ---- rotator tables: FP64 phase arguments, FP32 rotators
-        {
-#pragma clang fp contract(off)
-            const int integerdelay = (int)d.b;
-            const double bfrac = d.b - integerdelay;
-            const double lofr = p.lofrac * (double)integerdelay;
-            const float fd32 = (float)d.fd;
-            for (int e = tid; e < Gm::NTAB; e += NT) {
-                float arg;
-                bool unit = false;
-                float2* dst;
-                if (e < SUB) {
-                    double subxval = ((double)e / (double)F) * d.a;
-                    subxval += bfrac;
-                    double ph = subxval * p.lo;
-                    if (p.fracLO) ph += lofr;
-                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
-                    dst = rsub + e;
-                } else if (e < SUB + NSTEP) {
-                    const int k = e - SUB;
-                    const double stepxval = (((double)k * (double)STRIDE) / (double)F) * d.a;
-                    const double ph = stepxval * p.lo;
-                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
-                    unit = (k == 0);
-                    dst = rstep + k;
-                } else if (e < SUB + NSTEP + STRIDE) {
-                    const int s = e - SUB - NSTEP;
-                    arg = args.subchf[s] * fd32;
-                    dst = frsub + s;
-                } else {
-                    const int k = e - SUB - NSTEP - STRIDE;
-                    arg = args.stepchf[k] * fd32;
-                    unit = (k == 0);
-                    dst = frstep + k;
-                }
-                *dst = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
-            }
-        }
-        // ---- packed byte window of this FFT
-        const int o = d.offset;
+        // ---- tables and window of this FFT from registers to LDS
+        const int o = cur.offset;
         const int bbase = (o >> 1) & ~3;
-        const int nd = ((((o + F - 1) >> 1) - bbase) >> 2) + 1;
-        const uint32_t* src32 = reinterpret_cast<const uint32_t*>(src + bbase);
-        for (int w = tid; w < nd; w += NT) bytes[w] = src32[w];
+#pragma unroll
+        for (int t = 0; t < TPT; t++) {
+            const int e = tid + t * NT;
+            if (e < Gm::NTAB) tab[e] = cur.t[t];
+        }
+#pragma unroll
+        for (int w = 0; w < WPT; w++) {
+            const int k = tid + w * NT;
+            if (k < F / 8 + 4) bytes[k] = cur.w[w];
+        }
         __syncthreads();
+        if (gi + 1 < G) fetch(gi + 1, cur);
 
         // ---- pass 0: unpack + fringe rotate + radix-R0 butterfly
         {
@@ -569,7 +632,7 @@ void fx_fstage_kernel(FStageArgs args)
                 const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
                 const float2 y = cmul(fr, x[r]);
                 float si = 1.0f;
-                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + sig];
+                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
                 outr.put_last(m * (RL / 2) + r, y, si);
                 if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
             }
@@ -656,8 +719,12 @@ void fx_fstage_eo_kernel(FStageArgs args)
 
     __shared__ __align__(16) float2 buf[FP];
     __shared__ __align__(16) float lut[2][256];
-    __shared__ float2 rsub[SUB], rstep[NSTEP], frsub[STRIDE], frstep[NFSTEP];
+    __shared__ __align__(16) float2 tab[NTAB];
     __shared__ float2 tw1[NTW1];
+    const float2* rsub = tab;
+    const float2* rstep = tab + SUB;
+    const float2* frsub = tab + SUB + NSTEP;
+    const float2* frstep = tab + SUB + NSTEP + STRIDE;
 
     const FxPlan& p = args.p;
     const int nblocks = gridDim.x;
@@ -702,41 +769,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
             continue;
         }
         {
-#pragma clang fp contract(off)
-            const int integerdelay = (int)d.b;
-            const double bfrac = d.b - integerdelay;
-            const double lofr = p.lofrac * (double)integerdelay;
-            const float fd32 = (float)d.fd;
-            for (int e = tid; e < NTAB; e += NT) {
-                float arg;
-                bool unit = false;
-                float2* dst;
-                if (e < SUB) {
-                    double subxval = ((double)e / (double)F) * d.a;
-                    subxval += bfrac;
-                    double ph = subxval * p.lo;
-                    if (p.fracLO) ph += lofr;
-                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
-                    dst = rsub + e;
-                } else if (e < SUB + NSTEP) {
-                    const int k = e - SUB;
-                    const double stepxval = (((double)k * (double)STRIDE) / (double)F) * d.a;
-                    const double ph = stepxval * p.lo;
-                    arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
-                    unit = (k == 0);
-                    dst = rstep + k;
-                } else if (e < SUB + NSTEP + STRIDE) {
-                    const int s = e - SUB - NSTEP;
-                    arg = args.subchf[s] * fd32;
-                    dst = frsub + s;
-                } else {
-                    const int k = e - SUB - NSTEP - STRIDE;
-                    arg = args.stepchf[k] * fd32;
-                    unit = (k == 0);
-                    dst = frstep + k;
-                }
-                *dst = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
-            }
+            const float2* tb = args.rtab + ((long long)fft * p.nant + ant) * NTAB;
+            for (int e = tid; e < NTAB; e += NT) tab[e] = tb[e];
         }
         __syncthreads();
 
@@ -792,7 +826,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
                         const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
                         const float2 y = cmul(fr, X);
                         float si = 1.0f;
-                        if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)c * p.Spad + sig];
+                        if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
                         store_slot<PREC>(args, xrow0 + (long long)c * args.kc * p.Spad, gi, y, si);
                         if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
                     }

@@ -11,6 +11,7 @@ struct FStageArgs {
     const uint8_t* data;          // packed input, antenna a at data + a*data_stride
     long long data_stride;
     const FxDelay* delays;        // [nffts][nant] for this call
+    const float2* rtab;           // [nffts][nant][sub + nstep + stride + nfstep] rotator tables
     const float2* tw;             // [F] exp(-2 pi i m / F)
     const float* subchf;          // [stride] f32 channel frequencies (kernel.c:164)
     const float* stepchf;         // [nfstep] (kernel.c:165)
@@ -18,7 +19,7 @@ struct FStageArgs {
     int kc;                       // groups per X-engine K chunk
     int fg0;                      // first group written by this launch
     int ngroups;                  // groups in this launch
-    const float* scale_inv;       // FP8: [nchan][Spad] inverse scales
+    const float* scale_inv;       // FP8: [Spad][nchan] inverse scales (channel fastest: coalesced)
     float2* tap;                  // optional FP32 spectra [nffts][nant][2][nchan]
     int plan;                     // radix plan, 0 .. fstage_plans(lg) - 1
 };
@@ -37,6 +38,8 @@ struct XArgs {
 int fx_fstage_group_size(int prec);
 hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
                         int numffts, int nffts_pad, int maxoffset, FxDelay* out, hipStream_t s);
+hipError_t launch_rotators(const FxPlan& p, const FxDelay* dtab, int nffts, const float* subchf,
+                           const float* stepchf, float2* rtab, hipStream_t s);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
 bool fstage_supported(int lg);
 int fstage_plans(int lg);

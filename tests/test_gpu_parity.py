@@ -259,3 +259,24 @@ def test_autotune_and_cache(tmp_path, monkeypatch):
     monkeypatch.setenv("FX_TUNE_CACHE", cache)
     with _engine(p, nant, nf, prec="f32", data=data) as e2:
         assert (e2.plan(), e2.xtile(), e2.xbatch()) == (rep["plan"], rep["xtile"], rep["xbatch"])
+
+
+def test_fp8_per_channel_signal_scales():
+    # non-uniform scales: the F-stage divides by s[c][sig] and the X-engine
+    # multiplies by s_i s_j; a transposed or misread table breaks the bound
+    nchan, nant, nf = 1024, 5, 64
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 83, data="correlated")
+    rng = np.random.default_rng(84)
+    sigma = np.sqrt(2 * nchan * 4.3128 / 2.0)
+    scales = (sigma / 16.0) * rng.uniform(0.5, 2.0, (nchan, 2 * nant)).astype(np.float32)
+    with _engine(p, nant, nf, prec="f8", data=data) as e:
+        e.set_fp8_scales(scales)
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f64(p, data, delays, fileoff, it)
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= 0.1
+    assert _rel(autos[:, [0, 3]], ref["autos"][:, [0, 3]]).max() <= 0.02
