@@ -53,7 +53,12 @@ struct fx_context {
     int max_ffts;       // padded FFT capacity
     long long cap_bytes, stride;
     hipStream_t stream;
-    uint8_t* d_data;
+    uint8_t* d_data;    // input slot 0 (also the buffer of fx_upload_data / fx_fstage)
+    uint8_t* d_data1;   // input slot 1 (streaming ingest), allocated on first use
+    uint8_t* h_pin[2];  // pinned host staging of the two slots, allocated on first use
+    hipStream_t cstream;              // copy stream of the streaming ingest
+    hipEvent_t copied[2], slot_free[2];
+    int ingest;         // streaming ingest in use: F-stages wait for their slot's copy
     double* d_model;    // delays [nant*3] then fileoffsets [nant]
     int* d_iter;
     FxDelay* d_dtab;
@@ -96,7 +101,14 @@ extern "C" int fx_fft_granule(const fx_context* ctx) { return ctx ? ctx->granule
 
 static void free_ctx(fx_context* c)
 {
-    void* bufs[] = {c->d_data, c->d_model, c->d_iter, c->d_dtab, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
+    if (c->cstream) (void)hipStreamSynchronize(c->cstream);
+    for (int k = 0; k < 2; k++) {
+        if (c->h_pin[k]) (void)hipHostFree(c->h_pin[k]);
+        if (c->copied[k]) (void)hipEventDestroy(c->copied[k]);
+        if (c->slot_free[k]) (void)hipEventDestroy(c->slot_free[k]);
+    }
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
+    void* bufs[] = {c->d_data, c->d_data1, c->d_model, c->d_iter, c->d_dtab, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
                     c->d_xin, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv,
                     c->d_cross, c->d_autos, c->d_tap};
     for (void* b : bufs)
@@ -344,10 +356,11 @@ static int ev_collect(fx_context* c)
 // Host-side checks of one F-stage call; allocates the tap buffer when needed
 // (never inside a graph capture).
 struct FStageCall {
-    int numffts, npad, maxoffset, fft0;
+    int numffts, npad, maxoffset, fft0, origin, slot;
 };
 
-static int fstage_check(fx_context* c, int numffts, const int* iteration, int split, int fft0, FStageCall* f)
+static int fstage_check(fx_context* c, int numffts, const int* iteration, int split, int fft0, FStageCall* f,
+                        int origin = 0, int slot = 0)
 {
     if (!c || !iteration) return fail(FX_EINVAL, "null argument");
     if (numffts < 1 || split < 1) return fail(FX_EINVAL, "numffts=%d split=%d", numffts, split);
@@ -362,9 +375,13 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
     const long long need = ((long long)numffts * split * p.F + 1) / 2;
     if (need > c->cap_bytes)
         return fail(FX_EINVAL, "input needs %lld bytes per antenna, capacity %lld", need, c->cap_bytes);
-    for (int k = 0; k < numffts; k++)
-        if (iteration[k] < 0 || (long long)iteration[k] * p.F > 0x7fffffffLL - p.F)
-            return fail(FX_EINVAL, "iteration[%d]=%d out of range", k, iteration[k]);
+    if (slot < 0 || slot > 1 || (slot == 1 && !c->d_data1))
+        return fail(FX_EINVAL, "input slot %d not set up (fx_ingest_host first)", slot);
+    for (int k = 0; k < numffts; k++) {
+        const long long rel = ((long long)iteration[k] - origin) * p.F;
+        if (iteration[k] < 0 || rel > 0x7fffffffLL - p.F || rel < -0x7fffffffLL + p.F)
+            return fail(FX_EINVAL, "iteration[%d]=%d out of range (origin %d)", k, iteration[k], origin);
+    }
     if (c->tap && !c->d_tap) {
         if (hipMalloc((void**)&c->d_tap, sizeof(float2) * (size_t)c->max_ffts * p.nant * 2 * p.nchan) != hipSuccess)
             return fail(FX_ENOMEM, "tap allocation failed");
@@ -373,6 +390,8 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
     f->npad = npad;
     f->maxoffset = (int)maxoffset;
     f->fft0 = fft0;
+    f->origin = origin;
+    f->slot = slot;
     return FX_OK;
 }
 
@@ -380,12 +399,16 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
 static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
 {
     const FxPlan& p = c->p;
+    // the slot's input has arrived (graphs of batches record kernels only and
+    // read slot 0, filled by plain uploads)
+    const bool ingest = c->ingest && !c->capturing;
+    if (ingest) HIPCHK(hipStreamWaitEvent(s, c->copied[f.slot], 0));
     TIMED(TM_DELAY, s, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, f.numffts, f.npad,
-                                    f.maxoffset, c->d_dtab, s));
+                                    f.maxoffset, f.origin, c->d_dtab, s));
     TIMED(TM_DELAY, s, launch_rotators(p, c->d_dtab, f.npad, c->d_subchf, c->d_stepchf, c->d_rtab, s));
     FStageArgs a;
     a.p = p;
-    a.data = c->d_data;
+    a.data = f.slot ? c->d_data1 : c->d_data;
     a.data_stride = c->stride;
     a.delays = c->d_dtab;
     a.rtab = c->d_rtab;
@@ -400,6 +423,7 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     a.tap = c->tap ? c->d_tap : nullptr;
     a.plan = c->fplan;
     TIMED(TM_FSTAGE, s, launch_fstage(a, c->fprec, s));
+    if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], s));           // the slot may be overwritten
     TIMED(TM_COUNTS, s, launch_counts(p.nant, f.npad, c->d_dtab, c->d_counts, s));
     if (c->timing && !c->capturing) c->nl[0]++;
     c->last_nffts = f.npad;
@@ -415,6 +439,78 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
     rc = fstage_enqueue(c, f, c->stream);
     if (rc) return rc;
     if (nslots) *nslots = f.npad;
+    return FX_OK;
+}
+
+extern "C" int fx_fstage_ex(fx_context* c, int slot, int numffts, const int* iteration, int split, int origin,
+                            int fft0, int* nslots)
+{
+    FStageCall f;
+    int rc = fstage_check(c, numffts, iteration, split, fft0, &f, origin, slot);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+    rc = fstage_enqueue(c, f, c->stream);
+    if (rc) return rc;
+    if (nslots) *nslots = f.npad;
+    return FX_OK;
+}
+
+// Streaming ingest: two input slots, each a device buffer and a pinned host
+// staging buffer; H2D copies run on their own stream and overlap the
+// F-stage / X-engine work on the context's stream.
+static int ingest_setup(fx_context* c)
+{
+    if (c->ingest) return FX_OK;
+    const size_t bytes = (size_t)c->stride * c->p.nant;
+    if (!c->d_data1 && hipMalloc((void**)&c->d_data1, bytes) != hipSuccess)
+        return fail(FX_ENOMEM, "second input slot (%zu bytes) failed", bytes);
+    for (int k = 0; k < 2; k++)
+        if (!c->h_pin[k] && hipHostMalloc((void**)&c->h_pin[k], bytes, hipHostMallocDefault) != hipSuccess)
+            return fail(FX_ENOMEM, "pinned staging buffer (%zu bytes) failed", bytes);
+    if (!c->cstream) HIPCHK(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; k++) {
+        if (!c->copied[k]) HIPCHK(hipEventCreateWithFlags(&c->copied[k], hipEventDisableTiming));
+        if (!c->slot_free[k]) HIPCHK(hipEventCreateWithFlags(&c->slot_free[k], hipEventDisableTiming));
+        // both slots start free and filled
+        HIPCHK(hipEventRecord(c->copied[k], c->cstream));
+        HIPCHK(hipEventRecord(c->slot_free[k], c->stream));
+    }
+    c->ingest = 1;
+    return FX_OK;
+}
+
+extern "C" int fx_ingest_host(fx_context* c, int slot, uint8_t** ptr, long long* stride)
+{
+    if (!c || !ptr || !stride) return fail(FX_EINVAL, "null argument");
+    if (slot < 0 || slot > 1) return fail(FX_EINVAL, "slot %d (0 or 1)", slot);
+    int rc = ingest_setup(c);
+    if (rc) return rc;
+    *ptr = c->h_pin[slot];
+    *stride = c->stride;
+    return FX_OK;
+}
+
+extern "C" int fx_ingest_submit(fx_context* c, int slot, long long nbytes)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    if (!c->ingest) return fail(FX_EINVAL, "fx_ingest_host must come first");
+    if (slot < 0 || slot > 1) return fail(FX_EINVAL, "slot %d (0 or 1)", slot);
+    if (nbytes < 0 || nbytes > c->cap_bytes)
+        return fail(FX_EINVAL, "nbytes=%lld exceeds capacity %lld", nbytes, c->cap_bytes);
+    uint8_t* dst = slot ? c->d_data1 : c->d_data;
+    HIPCHK(hipStreamWaitEvent(c->cstream, c->slot_free[slot], 0));       // last F-stage on the slot is done
+    HIPCHK(hipMemcpy2DAsync(dst, (size_t)c->stride, c->h_pin[slot], (size_t)c->stride, (size_t)nbytes,
+                            (size_t)c->p.nant, hipMemcpyHostToDevice, c->cstream));
+    HIPCHK(hipEventRecord(c->copied[slot], c->cstream));
+    return FX_OK;
+}
+
+extern "C" int fx_ingest_wait(fx_context* c, int slot)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    if (!c->ingest) return FX_OK;
+    if (slot < 0 || slot > 1) return fail(FX_EINVAL, "slot %d (0 or 1)", slot);
+    HIPCHK(hipEventSynchronize(c->copied[slot]));
     return FX_OK;
 }
 

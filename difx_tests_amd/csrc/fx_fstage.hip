@@ -28,7 +28,7 @@ static constexpr double kPi = 3.14159265358979323846;
 
 __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
                                 const double* __restrict__ fileoff, const int* __restrict__ iteration,
-                                int numffts, int nffts_pad, int maxoffset, FxDelay* __restrict__ out)
+                                int numffts, int nffts_pad, int maxoffset, int origin, FxDelay* __restrict__ out)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,11 +50,12 @@ __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
     const double a = d2 - d0;
     const double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
     const double mean = a * 0.5 + b;
-    // kernel.c:736-756
+    // kernel.c:736-756; the offset is relative to a buffer whose first FFT is
+    // `origin` (0 for the reference's single-subint buffer)
     const double x = (mean - fileoff[j]) / p.Ts;
     const int n = (int)(x + 0.5);
     double fd = -(x - n) * p.Ts;
-    int o = fftindex * p.F - n;
+    int o = (fftindex - origin) * p.F - n;
     if (o == -1) {
         ++o;
         fd += p.Ts;
@@ -72,12 +73,12 @@ __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
 }
 
 hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
-                        int numffts, int nffts_pad, int maxoffset, FxDelay* out, hipStream_t s)
+                        int numffts, int nffts_pad, int maxoffset, int origin, FxDelay* out, hipStream_t s)
 {
     const int n = nffts_pad * p.nant;
     if (n <= 0) return hipSuccess;
     fx_delay_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, delays, fileoff, iteration, numffts, nffts_pad,
-                                                    maxoffset, out);
+                                                    maxoffset, origin, out);
     return hipGetLastError();
 }
 

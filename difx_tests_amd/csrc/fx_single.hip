@@ -324,7 +324,7 @@ void stationDelayAndOffset(double** delays, int* iteration, double* antFileOffse
     h2d(model.p, m, sizeof(m));
     h2d(it.p, iteration, sizeof(int));
     CK(fx::launch_delay(p, model.as<double>(), model.as<double>() + 3, it.as<int>(), 1, 1,
-                        (numffts - 1) * *fftchannels, tab.as<FxDelay>(), 0));
+                        (numffts - 1) * *fftchannels, 0, tab.as<FxDelay>(), 0));
     FxDelay e;
     d2h(&e, tab.p, sizeof(e));
     *delaya = e.a;
@@ -358,7 +358,7 @@ void processAntennas(FxKernel* kernel, int* nant, int* numffts, int* fftchannels
     h2d(model.p, m.data(), sizeof(double) * m.size());
     h2d(it.p, &i, sizeof(int));
     CK(fx::launch_delay(p, model.as<double>(), model.as<double>() + 3 * *nant, it.as<int>(), 1, 1,
-                        (*numffts - 1) * F, tab.as<FxDelay>(), 0));
+                        (*numffts - 1) * F, 0, tab.as<FxDelay>(), 0));
     std::vector<FxDelay> e((size_t)*nant);
     d2h(e.data(), tab.p, sizeof(FxDelay) * *nant);
     for (int j = 0; j < *nant; j++) {

@@ -24,7 +24,7 @@ NATIVE_SYMBOLS = [
     "fx_read_vis", "fx_read_delay_table", "fx_enable_tap", "fx_read_tap", "fx_set_fp8_scales", "fx_sync",
     "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry", "fx_set_option", "fx_get_option",
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
-    "fx_autotune",
+    "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait",
 ]
 OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH = 1, 2, 3, 4
 BATCH_LAUNCH, BATCH_GRAPH = 0, 1
@@ -95,6 +95,10 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_batch_sync.argtypes = [P]
     lib.fx_batch_captures.argtypes = [P]
     lib.fx_autotune.argtypes = [P, I, I, C.c_char_p, P]
+    lib.fx_fstage_ex.argtypes = [P, I, I, P, I, I, I, C.POINTER(I)]
+    lib.fx_ingest_host.argtypes = [P, I, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_longlong)]
+    lib.fx_ingest_submit.argtypes = [P, I, C.c_longlong]
+    lib.fx_ingest_wait.argtypes = [P, I]
     _lib = lib
     return lib
 
@@ -161,6 +165,28 @@ class FxEngine:
         ns = C.c_int()
         self._check(self.lib.fx_fstage(self.h, len(it), _ptr(it), split, fft0, C.byref(ns)))
         return ns.value
+
+    def fstage_ex(self, iteration, slot: int = 0, split: int = 1, origin: int = 0, fft0: int = 0) -> int:
+        """F-stage on input slot `slot` of a buffer whose first FFT is `origin`
+        (global FFT indices in `iteration`)."""
+        it = np.ascontiguousarray(iteration, dtype=np.int32)
+        ns = C.c_int()
+        self._check(self.lib.fx_fstage_ex(self.h, slot, len(it), _ptr(it), split, origin, fft0, C.byref(ns)))
+        return ns.value
+
+    # ---- streaming ingest
+    def ingest_host(self, slot: int) -> np.ndarray:
+        """Pinned host buffer of an input slot as uint8 [nant, stride]."""
+        ptr = C.POINTER(C.c_uint8)()
+        stride = C.c_longlong()
+        self._check(self.lib.fx_ingest_host(self.h, slot, C.byref(ptr), C.byref(stride)))
+        return np.ctypeslib.as_array(ptr, shape=(self.nant, stride.value))
+
+    def ingest_submit(self, slot: int, nbytes: int) -> None:
+        self._check(self.lib.fx_ingest_submit(self.h, slot, nbytes))
+
+    def ingest_wait(self, slot: int) -> None:
+        self._check(self.lib.fx_ingest_wait(self.h, slot))
 
     def xstage(self, fft0: int, nslots: int) -> None:
         self._check(self.lib.fx_xstage(self.h, fft0, nslots))

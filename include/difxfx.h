@@ -76,6 +76,31 @@ int fx_device_data(fx_context *ctx, void **dptr, long long *stride);
  * FFT slots [fft0, fft0 + padded numffts); fft0 must be a multiple of the
  * granule.  *nslots receives the number of slots written. */
 int fx_fstage(fx_context *ctx, int numffts, const int *iteration, int split, int fft0, int *nslots);
+/* fx_fstage on input slot `slot` (0: the buffer of fx_upload_data; 1: the
+ * second streaming slot) for a buffer whose first sample is that of global
+ * FFT `origin`: the sample offset of FFT g is (g - origin)*F - n, with the
+ * edge clamps and validity of kernel.c:742-756 applied at the buffer's own
+ * edges.  origin 0 is the reference (one subint buffer starting at FFT 0);
+ * a streaming caller passes global FFT indices and origin = the subint's
+ * first FFT, so delay polynomials keep one time origin across subints. */
+int fx_fstage_ex(fx_context *ctx, int slot, int numffts, const int *iteration, int split, int origin, int fft0,
+                 int *nslots);
+
+/* Streaming ingest: two input slots, each a device buffer of the context's
+ * input capacity and a pinned host staging buffer of the same layout.
+ *   fx_ingest_host    pinned host memory of a slot; antenna a's bytes go at
+ *                     ptr + a*stride (the caller fills it, e.g. from a file)
+ *   fx_ingest_submit  asynchronous H2D of the slot on the context's copy
+ *                     stream, after the last F-stage that read the slot
+ *   fx_ingest_wait    blocks until the slot's copy is complete, so its host
+ *                     buffer may be refilled
+ * An F-stage on a slot (fx_fstage_ex) waits for that slot's latest copy.
+ * With two slots the copy of subint s+1 overlaps the F-stage and X-engine of
+ * subint s. */
+int fx_ingest_host(fx_context *ctx, int slot, uint8_t **ptr, long long *stride);
+int fx_ingest_submit(fx_context *ctx, int slot, long long nbytes);
+int fx_ingest_wait(fx_context *ctx, int slot);
+
 /* X-engine over FFT slots [fft0, fft0 + nslots), accumulated into the
  * context's visibilities. */
 int fx_xstage(fx_context *ctx, int fft0, int nslots);

@@ -43,6 +43,7 @@ int or_plan_init(or_plan *p, int nchan, double lo, double bw)
     p->lo = lo;
     p->bw = bw;
     p->fracLO = (lo - (double)(int)lo) > OR_TINY; /* kernel.c:39 */
+    p->origin = 0;
     return 0;
 }
 
@@ -65,13 +66,15 @@ void or_station_delay(const double c[3], int fftindex, double *mean, double *a, 
 }
 
 /* kernel.c:736-756 */
-int or_offset(double mean, double fileoffset, double Ts, int g, int F, int maxoffset,
-              int *offset, double *fracdelay)
+/* kernel.c:736-756 with the sample offset taken relative to a buffer that
+ * starts at FFT `origin` (0 for the reference). */
+int or_offset_origin(double mean, double fileoffset, double Ts, int g, int origin, int F, int maxoffset,
+                     int *offset, double *fracdelay)
 {
     double x = (mean - fileoffset) / Ts;
     int n = (int)(x + 0.5);               /* truncation toward zero (Q3) */
     double fd = -(x - n) * Ts;
-    int o = g * F - n;
+    int o = (g - origin) * F - n;
     if (o == -1) {
         ++o;
         fd += Ts;
@@ -83,6 +86,12 @@ int or_offset(double mean, double fileoffset, double Ts, int g, int F, int maxof
     *offset = o;
     *fracdelay = fd;
     return !(o < 0 || o > maxoffset);
+}
+
+int or_offset(double mean, double fileoffset, double Ts, int g, int F, int maxoffset,
+              int *offset, double *fracdelay)
+{
+    return or_offset_origin(mean, fileoffset, Ts, g, 0, F, maxoffset, offset, fracdelay);
 }
 
 /* kernel.c:191-192 */
@@ -335,7 +344,7 @@ static int fstage_one_f32(const or_plan *p, const double *c, double fileoffset, 
     double mean, a, b, fd;
     int offset;
     or_station_delay(c, g, &mean, &a, &b);
-    int valid = or_offset(mean, fileoffset, p->Ts, g, F, maxoffset, &offset, &fd);
+    int valid = or_offset_origin(mean, fileoffset, p->Ts, g, p->origin, F, maxoffset, &offset, &fd);
     if (offset_out) *offset_out = offset;
     if (fd_out) *fd_out = fd;
     if (!valid) return 0;
@@ -570,7 +579,8 @@ int or_process_f64(const or_plan *p, int nant, int numffts, int split,
                 double mean, a, b, fd;
                 int off;
                 or_station_delay(delays + 3 * j, iteration[k], &mean, &a, &b);
-                valid[j] = or_offset(mean, fileoffsets[j], p->Ts, iteration[k], F, maxoffset, &off, &fd);
+                valid[j] = or_offset_origin(mean, fileoffsets[j], p->Ts, iteration[k], p->origin, F, maxoffset,
+                                            &off, &fd);
                 if (valid_out) valid_out[(size_t)k * nant + j] = valid[j];
                 or_cf64 *sj = spec + (size_t)j * 2 * nchan;
                 if (valid[j]) {

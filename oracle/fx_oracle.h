@@ -53,6 +53,10 @@ typedef struct {
     double lo;      /* LO frequency (Hz)                                    */
     double bw;      /* bandwidth (Hz)                                       */
     int fracLO;     /* (lo - (int)lo) > 1e-11                               */
+    int origin;     /* global FFT index of the first FFT of the data buffer:
+                       o = (g - origin)*F - n.  0 (set by or_plan_init) is the
+                       reference, whose buffer starts at FFT 0; a streaming
+                       caller's subint buffers start at later FFTs.         */
 } or_plan;
 
 /* Returns 0 on success, -1 if nchan is not a power of two >= 4. */
@@ -63,6 +67,8 @@ void or_station_delay(const double c[3], int fftindex, double *mean, double *a, 
 
 /* kernel.c:736-756: integer/fractional split, edge clamps, validity.
  * Returns 1 if the antenna is valid for this FFT. */
+int or_offset_origin(double mean, double fileoffset, double Ts, int g, int origin, int F, int maxoffset,
+                     int *offset, double *fracdelay);
 int or_offset(double mean, double fileoffset, double Ts, int g, int F, int maxoffset,
               int *offset, double *fracdelay);
 

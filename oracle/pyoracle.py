@@ -18,7 +18,7 @@ SO = os.path.join(HERE, "_build", "libfxoracle.so")
 class OrPlan(C.Structure):
     _fields_ = [("nchan", C.c_int), ("F", C.c_int), ("stride", C.c_int), ("sub", C.c_int), ("nstep", C.c_int),
                 ("nfstep", C.c_int), ("Ts", C.c_double), ("lo", C.c_double), ("bw", C.c_double),
-                ("fracLO", C.c_int)]
+                ("fracLO", C.c_int), ("origin", C.c_int)]
 
 
 _lib = None
@@ -43,6 +43,7 @@ def lib() -> C.CDLL:
         L.or_plan_init.argtypes = [C.POINTER(OrPlan), I, D, D]
         L.or_station_delay.argtypes = [P, I, P, P, P]
         L.or_offset.argtypes = [D, D, D, I, I, I, P, P]
+        L.or_offset_origin.argtypes = [D, D, D, I, I, I, I, P, P]
         L.or_unpack2bit.argtypes = [P, I, I, P, P]
         L.or_level.argtypes = [I]
         L.or_level.restype = C.c_float
@@ -67,11 +68,13 @@ def _p(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
-def plan(nchan: int, lo: float, bw: float) -> OrPlan:
+def plan(nchan: int, lo: float, bw: float, origin: int = 0) -> OrPlan:
+    """origin: global FFT index of the data buffer's first FFT (0: the reference)."""
     p = OrPlan()
     rc = lib().or_plan_init(C.byref(p), nchan, lo, bw)
     if rc:
         raise ValueError(f"or_plan_init failed ({rc})")
+    p.origin = origin
     return p
 
 
@@ -161,9 +164,12 @@ def station_delay(c, g):
     return mean.value, a.value, b.value
 
 
-def offset(mean, fileoff, Ts, g, F, maxoffset):
+def offset(mean, fileoff, Ts, g, F, maxoffset, origin=None):
     o, fd = C.c_int(), C.c_double()
-    v = lib().or_offset(mean, fileoff, Ts, g, F, maxoffset, C.byref(o), C.byref(fd))
+    if origin is None:
+        v = lib().or_offset(mean, fileoff, Ts, g, F, maxoffset, C.byref(o), C.byref(fd))
+    else:
+        v = lib().or_offset_origin(mean, fileoff, Ts, g, origin, F, maxoffset, C.byref(o), C.byref(fd))
     return v, o.value, fd.value
 
 

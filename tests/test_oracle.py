@@ -320,3 +320,63 @@ def test_fstage_threads_match_single_thread():
     a, va = orc.fstage_f32(p, data, delays, fileoff, np.arange(nf), nthreads=1)
     b, vb = orc.fstage_f32(p, data, delays, fileoff, np.arange(nf), nthreads=4)
     assert np.array_equal(a, b) and np.array_equal(va, vb)
+
+
+# ---- buffer origin (streaming subints; origin 0 is the reference)
+
+def test_offset_origin_zero_is_reference():
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        mean, fo = rng.uniform(-3e-6, 3e-6, 2)
+        g = int(rng.integers(0, 50))
+        a = orc.offset(mean, fo, 1 / 64e6, g, 2048, 49 * 2048)
+        b = orc.offset(mean, fo, 1 / 64e6, g, 2048, 49 * 2048, origin=0)
+        assert a == b
+
+
+def test_offset_origin_shifts_window_and_edges():
+    # the same net delay relative to a buffer starting at FFT 7: the offset is
+    # 7*F smaller and the clamps act at the buffer's own edges
+    Ts, F, nf = 1 / 64e6, 2048, 10
+    maxo = (nf - 1) * F
+    for g in range(7, 7 + nf):
+        for x in (-0.4, 0.6, -1.0, 1.4):           # net delay in samples
+            mean = x * Ts
+            v0, o0, fd0 = orc.offset(mean, 0.0, Ts, g, F, (g + 1) * F * 4)
+            v1, o1, fd1 = orc.offset(mean, 0.0, Ts, g, F, maxo, origin=7)
+            n = int(x + 0.5)
+            if g == 7 and o0 - 7 * F == -1:
+                assert (v1, o1) == (1, 0) and fd1 == fd0 + Ts       # low-edge clamp of the buffer
+            elif g == 7 + nf - 1 and o0 - 7 * F == maxo + 1:
+                assert (v1, o1) == (1, maxo) and fd1 == fd0 - Ts    # high-edge clamp
+            else:
+                assert o1 == o0 - 7 * F and fd1 == fd0 and o1 == (g - 7) * F - n
+                assert v1 == (0 <= o1 <= maxo)
+
+
+def test_streaming_subints_equal_whole_stream_in_the_interior():
+    # three subints cut from one stream (each buffer holds NF + 1 FFTs of
+    # samples, allocDataHost.c:21) with global FFT indices and origins give the
+    # whole-stream spectra bit for bit wherever both place the window alike
+    nchan, nant, nf, S = 256, 3, 6, 3
+    F = 2 * nchan
+    p0 = orc.plan(nchan, 1.65e9, 16e6)
+    rng = np.random.default_rng(12)
+    delays = np.stack([np.zeros(nant), rng.uniform(-2e-10, 2e-10, nant), rng.uniform(-1e-6, 1e-6, nant)], axis=1)
+    fileoff = delays[:, 2] + rng.uniform(-0.45, 0.45, nant) * p0.Ts
+    stream = synth.random_bytes(nant, (S * nf + 1) * F // 2, 12)
+    whole = orc.process_f32(p0, stream, delays, fileoff, np.arange(S * nf, dtype=np.int32), want_chan=True)
+    compared = 0
+    for s in range(S):
+        ps = orc.plan(nchan, 1.65e9, 16e6, origin=s * nf)
+        buf = np.ascontiguousarray(stream[:, s * nf * F // 2: (s * nf + nf + 1) * F // 2])
+        it = (s * nf + np.arange(nf)).astype(np.int32)
+        part = orc.process_f32(ps, buf, delays, fileoff, it, want_chan=True)
+        for k in range(nf):
+            g = s * nf + k
+            for j in range(nant):
+                same = part["offset"][k, j] + s * nf * F == whole["offset"][g, j]
+                if part["valid"][k, j] and whole["valid"][g, j] and same:
+                    assert np.array_equal(part["chan"][k, j], whole["chan"][g, j])
+                    compared += 1
+    assert compared >= S * nf * nant - 2 * S * nant
