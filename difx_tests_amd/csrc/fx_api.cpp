@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "difxfx.h"
@@ -563,12 +565,15 @@ extern "C" int fx_reset(fx_context* c)
     return FX_OK;
 }
 
-extern "C" int fx_read_vis(fx_context* c, float* cross, int* cross_counts, float* autos, int* auto_counts)
+extern "C" int fx_read_vis_ex(fx_context* c, int flags, float* cross, int* cross_counts, float* autos,
+                              int* auto_counts)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
+    if (flags & ~FX_VIS_NORMALIZE) return fail(FX_EINVAL, "unknown flags 0x%x", flags);
     const FxPlan& p = c->p;
     const int nbl = p.nant * (p.nant - 1) / 2;
-    HIPCHK(launch_export(p, c->d_vacc, c->d_cross, c->d_autos, c->stream));
+    HIPCHK(launch_export(p, c->d_vacc, c->d_cross, c->d_autos, (flags & FX_VIS_NORMALIZE) ? c->d_counts : nullptr,
+                         c->stream));
     if (cross && nbl)
         HIPCHK(hipMemcpyAsync(cross, c->d_cross, sizeof(float2) * (size_t)nbl * 4 * p.nchan, hipMemcpyDeviceToHost,
                               c->stream));
@@ -586,6 +591,82 @@ extern "C" int fx_read_vis(fx_context* c, float* cross, int* cross_counts, float
     if (auto_counts)
         for (int j = 0; j < p.nant; j++) auto_counts[j] = cnt[(size_t)j * p.nant + j];
     return FX_OK;
+}
+
+extern "C" int fx_read_vis(fx_context* c, float* cross, int* cross_counts, float* autos, int* auto_counts)
+{
+    return fx_read_vis_ex(c, 0, cross, cross_counts, autos, auto_counts);
+}
+
+// Visibility files.  Binary layout (little endian):
+//   char magic[8] = "DIFXFXV1"; int32 nant, nchan, nbl, flags; double bw, lo;
+//   int32 cross_counts[nbl]; int32 auto_counts[nant];
+//   float cross[nbl][4][nchan][2]; float autos[nant][4][nchan][2]
+// Text: the reference's vis.out (kernel.c:608-617) of the cross visibilities,
+// one line per channel, lines formatted in parallel.
+static int write_text(const char* path, const FxPlan& p, const std::vector<float2>& cross)
+{
+    const int nbl = p.nant * (p.nant - 1) / 2;
+    FILE* f = fopen(path, "w");
+    if (!f) return fail(FX_EINVAL, "cannot open %s", path);
+    const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int chunk = 64;                                  // channels formatted per round and thread
+    std::vector<std::string> lines((size_t)nth * chunk);
+    int rc = FX_OK;
+    for (int c0 = 0; c0 < p.nchan && rc == FX_OK; c0 += nth * chunk) {
+        const int c1 = std::min(p.nchan, c0 + nth * chunk);
+        auto work = [&](int t) {
+            char num[64];
+            for (int c = c0 + t; c < c1; c += nth) {
+                std::string& L = lines[c - c0];
+                L.clear();
+                snprintf(num, sizeof(num), "%5d %11.6f", c, (c + 0.5) / p.nchan * p.bw / 1e6);
+                L += num;
+                for (int b = 0; b < nbl; b++)
+                    for (int q = 0; q < 4; q++) {
+                        const float2 v = cross[((size_t)b * 4 + q) * p.nchan + c];
+                        const float amp = (float)sqrt((double)v.x * v.x + (double)v.y * v.y);
+                        const float ph = (float)atan2((double)v.y, (double)v.x);
+                        snprintf(num, sizeof(num), " %11.6f %11.6f %11.6f %10.6f", v.x, v.y, amp, ph);
+                        L += num;
+                    }
+                L += '\n';
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; t++) th.emplace_back(work, t);
+        work(0);
+        for (std::thread& x : th) x.join();
+        for (int c = c0; c < c1; c++)
+            if (fwrite(lines[c - c0].data(), 1, lines[c - c0].size(), f) != lines[c - c0].size()) rc = FX_EINVAL;
+    }
+    if (fclose(f) != 0 || rc) return fail(FX_EINVAL, "writing %s failed", path);
+    return FX_OK;
+}
+
+extern "C" int fx_save_vis(fx_context* c, const char* path, int format, int flags)
+{
+    if (!c || !path) return fail(FX_EINVAL, "null argument");
+    if (format != FX_VIS_BINARY && format != FX_VIS_TEXT) return fail(FX_EINVAL, "unknown format %d", format);
+    const FxPlan& p = c->p;
+    const int nbl = p.nant * (p.nant - 1) / 2;
+    std::vector<float2> cross((size_t)std::max(nbl, 1) * 4 * p.nchan), autos((size_t)p.nant * 4 * p.nchan);
+    std::vector<int> cc(std::max(nbl, 1)), ac(p.nant);
+    int rc = fx_read_vis_ex(c, flags, reinterpret_cast<float*>(cross.data()), cc.data(),
+                            reinterpret_cast<float*>(autos.data()), ac.data());
+    if (rc) return rc;
+    if (format == FX_VIS_TEXT) return write_text(path, p, cross);
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(FX_EINVAL, "cannot open %s", path);
+    const int32_t hdr[4] = {p.nant, p.nchan, nbl, flags};
+    const double fb[2] = {p.bw, p.lo};
+    bool ok = fwrite("DIFXFXV1", 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
+              fwrite(fb, sizeof(fb), 1, f) == 1 && fwrite(cc.data(), sizeof(int), nbl, f) == (size_t)nbl &&
+              fwrite(ac.data(), sizeof(int), p.nant, f) == (size_t)p.nant &&
+              fwrite(cross.data(), sizeof(float2), (size_t)nbl * 4 * p.nchan, f) == (size_t)nbl * 4 * p.nchan &&
+              fwrite(autos.data(), sizeof(float2), autos.size(), f) == autos.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? FX_OK : fail(FX_EINVAL, "writing %s failed", path);
 }
 
 extern "C" int fx_read_delay_table(fx_context* c, int nffts, void* out)

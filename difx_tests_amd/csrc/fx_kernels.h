@@ -51,7 +51,10 @@ int xengine_chunk_groups(int prec);
 int xengine_super_blocks(int nblk);
 hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s);
 hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s);
-hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, hipStream_t s);
+// counts (may be null): [nant*nant] valid-FFT counts; when given, the export
+// is normalised by them.
+hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, const int* counts,
+                         hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 
 }  // namespace fx

@@ -484,8 +484,10 @@ hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts
 
 // Block accumulators -> reference layout: cross[b][2l+m][c] for j<k in
 // j-major order (kernel.c:831-852) and autos[j][2l+m][c].
+// With counts, each visibility is divided by its valid-FFT count where that
+// is positive (processNormalize, kernel.c:348-359: (re / n, im / n)).
 __global__ void fx_export_kernel(FxPlan p, const float2* __restrict__ vacc, float2* __restrict__ cross,
-                                 float2* __restrict__ autos)
+                                 float2* __restrict__ autos, const int* __restrict__ counts)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nbl = p.nant * (p.nant - 1) / 2;
@@ -526,6 +528,14 @@ __global__ void fx_export_kernel(FxPlan p, const float2* __restrict__ vacc, floa
     const int bpi = fx_bp_index(I, J, p.nblk);
     float2 v = vacc[((long long)c * p.nbp + bpi) * (FX_BLK * FX_BLK) + (i1 % FX_BLK) * FX_BLK + (i2 % FX_BLK)];
     if (conj) v.y = -v.y;
+    if (counts) {
+        const int n = counts[j * p.nant + k];
+        if (n > 0) {
+            const float fn = (float)n;
+            v.x = v.x / fn;
+            v.y = v.y / fn;
+        }
+    }
     if (isauto) {
         if (autos) autos[t - ncross] = v;
     } else if (cross) {
@@ -533,11 +543,12 @@ __global__ void fx_export_kernel(FxPlan p, const float2* __restrict__ vacc, floa
     }
 }
 
-hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, hipStream_t s)
+hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, const int* counts,
+                         hipStream_t s)
 {
     const int nbl = p.nant * (p.nant - 1) / 2;
     const long long n = ((long long)nbl + p.nant) * 4 * p.nchan;
-    fx_export_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, vacc, cross, autos);
+    fx_export_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, vacc, cross, autos, counts);
     return hipGetLastError();
 }
 

@@ -24,8 +24,11 @@ NATIVE_SYMBOLS = [
     "fx_read_vis", "fx_read_delay_table", "fx_enable_tap", "fx_read_tap", "fx_set_fp8_scales", "fx_sync",
     "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry", "fx_set_option", "fx_get_option",
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
-    "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait",
+    "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait", "fx_read_vis_ex",
+    "fx_save_vis",
 ]
+VIS_NORMALIZE = 1
+VIS_TEXT, VIS_BINARY = 0, 1
 OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH = 1, 2, 3, 4
 BATCH_LAUNCH, BATCH_GRAPH = 0, 1
 # Every symbol include/kernel.h declares (kernel.h:61-134 of the reference).
@@ -99,6 +102,8 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_ingest_host.argtypes = [P, I, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_longlong)]
     lib.fx_ingest_submit.argtypes = [P, I, C.c_longlong]
     lib.fx_ingest_wait.argtypes = [P, I]
+    lib.fx_read_vis_ex.argtypes = [P, I, P, P, P, P]
+    lib.fx_save_vis.argtypes = [P, C.c_char_p, I, I]
     _lib = lib
     return lib
 
@@ -202,14 +207,21 @@ class FxEngine:
         self._check(self.lib.fx_sync(self.h))
 
     # ---- outputs
-    def read_vis(self):
-        """(cross [nbl,4,nchan] c64, counts [nbl], autos [nant,4,nchan] c64, autocounts [nant])."""
+    def read_vis(self, normalize: bool = False):
+        """(cross [nbl,4,nchan] c64, counts [nbl], autos [nant,4,nchan] c64, autocounts [nant]);
+        normalize divides by the valid-FFT counts on the device."""
         cross = np.zeros((max(self.nbl, 1), 4, self.nchan), np.complex64)
         autos = np.zeros((self.nant, 4, self.nchan), np.complex64)
         cc = np.zeros(max(self.nbl, 1), np.int32)
         ac = np.zeros(self.nant, np.int32)
-        self._check(self.lib.fx_read_vis(self.h, _ptr(cross), _ptr(cc), _ptr(autos), _ptr(ac)))
+        self._check(self.lib.fx_read_vis_ex(self.h, VIS_NORMALIZE if normalize else 0, _ptr(cross), _ptr(cc),
+                                            _ptr(autos), _ptr(ac)))
         return cross[: self.nbl], cc[: self.nbl], autos, ac
+
+    def save_vis(self, path: str, fmt: str = "binary", normalize: bool = True) -> None:
+        """Writes the visibilities: fmt "binary" (DIFXFXV1) or "text" (vis.out)."""
+        self._check(self.lib.fx_save_vis(self.h, path.encode(), VIS_BINARY if fmt == "binary" else VIS_TEXT,
+                                         VIS_NORMALIZE if normalize else 0))
 
     def read_delay_table(self, nffts: int) -> np.ndarray:
         out = np.zeros(nffts * self.nant, DELAY_DTYPE)

@@ -116,6 +116,23 @@ int fx_reset(fx_context *ctx);
  *   autos[((a*4 + p)*nchan + c)*2 + {re,im}], auto_counts[a]
  * Any output pointer may be NULL. */
 int fx_read_vis(fx_context *ctx, float *cross, int *cross_counts, float *autos, int *auto_counts);
+/* Same with flags: FX_VIS_NORMALIZE divides every visibility by its
+ * valid-FFT count on the device (processNormalize, kernel.c:348-359;
+ * zero counts leave the visibility as it is). */
+#define FX_VIS_NORMALIZE 1
+int fx_read_vis_ex(fx_context *ctx, int flags, float *cross, int *cross_counts, float *autos, int *auto_counts);
+
+/* Visibility files of the accumulated visibilities (flags as fx_read_vis_ex).
+ * FX_VIS_TEXT    the reference's vis.out (kernel.c:570-630) of the cross
+ *                visibilities, one line per channel
+ * FX_VIS_BINARY  "DIFXFXV1", int32 nant, nchan, nbl, flags, double bw, lo,
+ *                int32 cross_counts[nbl], int32 auto_counts[nant],
+ *                float cross[nbl][4][nchan][2], float autos[nant][4][nchan][2]
+ *                (little endian; for large antenna counts, where the text
+ *                file would run to gigabytes) */
+#define FX_VIS_TEXT 0
+#define FX_VIS_BINARY 1
+int fx_save_vis(fx_context *ctx, const char *path, int format, int flags);
 
 /* Debug taps (host copies).  Delay table entries are 32 bytes:
  * {double a, b, fd; int offset, valid} per (FFT, antenna) of the last
