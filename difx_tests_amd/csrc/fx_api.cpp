@@ -137,18 +137,23 @@ static void tune_apply_cached(fx_context* c);
 static int make_plan(const fx_params* prm, FxPlan* p)
 {
     memset(p, 0, sizeof(*p));
-    const int lg = fx_ilog2_exact(2 * prm->nchan);
+    if (prm->iscomplex != 0 && prm->iscomplex != 1) return fail(FX_EINVAL, "iscomplex=%d", prm->iscomplex);
+    const int cplx = prm->iscomplex;
+    const int F = cplx ? prm->nchan : 2 * prm->nchan;
+    const int lg = fx_ilog2_exact(F);
     if (lg < 0) return fail(FX_EINVAL, "nchan=%d is not a power of two", prm->nchan);
-    if (!fstage_supported(lg))
-        return fail(FX_EUNSUPPORTED, "FFT length %d not supported (512..8192)", 2 * prm->nchan);
+    if (cplx ? !fstage_supported_complex(lg) : !fstage_supported(lg))
+        return fail(FX_EUNSUPPORTED, "%s FFT length %d not supported (%s)", cplx ? "complex" : "real", F,
+                    cplx ? "512..16384" : "512..32768");
     if (prm->nant < 1) return fail(FX_EINVAL, "nant=%d", prm->nant);
     if (!(prm->bw > 0)) return fail(FX_EINVAL, "bandwidth must be positive");
     if (!(prm->lo >= 0 && prm->lo < 2147483647.0)) return fail(FX_EINVAL, "LO must be in [0, 2^31) Hz");
     p->nant = prm->nant;
     p->nchan = prm->nchan;
-    p->F = 2 * prm->nchan;
+    p->F = F;
     p->lg = lg;
-    p->stride = 1 << ((lg - 1) / 2);
+    p->iscomplex = cplx;
+    p->stride = 1 << (fx_ilog2_exact(prm->nchan) / 2);   // 2^floor(log2(nchan)/2)
     p->sub = 2 * p->stride;
     p->nstep = p->F / p->sub;
     p->nfstep = p->nchan / p->stride;
@@ -156,7 +161,7 @@ static int make_plan(const fx_params* prm, FxPlan* p)
     p->Spad = (p->S + FX_BLK - 1) / FX_BLK * FX_BLK;
     p->nblk = p->Spad / FX_BLK;
     p->nbp = p->nblk * (p->nblk + 1) / 2;
-    p->Ts = 1.0 / (2.0 * prm->bw);
+    p->Ts = cplx ? 1.0 / prm->bw : 1.0 / (2.0 * prm->bw);
     p->lo = prm->lo;
     p->bw = prm->bw;
     p->lofrac = prm->lo - (double)(int)prm->lo;
@@ -189,7 +194,9 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     p.G = c->G;
     c->max_ffts = (prm->max_ffts + c->granule - 1) / c->granule * c->granule;
     c->nfg = c->max_ffts / c->G;
-    const long long need = (long long)p.nchan * (prm->max_ffts + 1); // allocDataHost.c:21
+    // allocDataHost.c:21 for real 2-bit data; complex 8-bit data take 4 bytes
+    // per sample (both polarisations' int8 re, im)
+    const long long need = p.iscomplex ? 4LL * p.F * (prm->max_ffts + 1) : (long long)p.nchan * (prm->max_ffts + 1);
     c->cap_bytes = prm->max_bytes_per_ant > need ? prm->max_bytes_per_ant : need;
     c->stride = (c->cap_bytes + 64 + 255) / 256 * 256;
 
@@ -239,7 +246,8 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     // FP8 default scales: the spectrum of 2-bit noise has E|Y|^2 = F * E[L^2]
     // with E[L^2] = 4.3128 for levels {+-1, +-3.3359} at +-0.98 sigma; map one
     // standard deviation of each component to 16.
-    const float sigma = (float)sqrt(p.F * 4.3128 / 2.0);
+    // complex input: the default assumes int8 components at 24 rms
+    const float sigma = (float)sqrt(p.iscomplex ? p.F * 2.0 * 24.0 * 24.0 / 2.0 : p.F * 4.3128 / 2.0);
     std::vector<float> sc((size_t)p.nchan * p.Spad, sigma / 16.0f), sci((size_t)p.nchan * p.Spad, 16.0f / sigma);
 
     if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * p.F, hipMemcpyHostToDevice) != hipSuccess ||
@@ -374,7 +382,7 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
     const long long maxoffset = ((long long)numffts * split - 1) * p.F;
     if (maxoffset > 0x7fffffffLL - 2LL * p.F) return fail(FX_EINVAL, "subint too long for 32-bit offsets");
     // The kernel reads samples [offset, offset + F) with offset <= maxoffset.
-    const long long need = ((long long)numffts * split * p.F + 1) / 2;
+    const long long need = p.iscomplex ? 4LL * numffts * split * p.F : ((long long)numffts * split * p.F + 1) / 2;
     if (need > c->cap_bytes)
         return fail(FX_EINVAL, "input needs %lld bytes per antenna, capacity %lld", need, c->cap_bytes);
     if (slot < 0 || slot > 1 || (slot == 1 && !c->d_data1))

@@ -38,6 +38,7 @@ struct FxPlan {
     int S, Spad, nblk, nbp;          // signals, padded signals, blocks, block pairs
     int G;                           // FFTs per corner-turn group
     int fracLO;
+    int iscomplex;                   // 0: real 2-bit input (F = 2*nchan); 1: complex 8-bit (F = nchan)
     double Ts, lo, bw, lofrac;       // lofrac = lo - (int)lo
 };
 

@@ -307,12 +307,14 @@ __host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : 2; }
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <int LG, int V = 0> struct Geo {
+// CPLX: complex input, F = nchan and every bin a channel; otherwise real
+// input, nchan = F/2.  stride = 2^floor(log2(nchan)/2) either way.
+template <int LG, int V = 0, bool CPLX = false> struct Geo {
     using PL_ = FPlan<LG, V>;
     static constexpr int LGV = LG;
     static constexpr int F = 1 << LG;
-    static constexpr int NCH = F / 2;
-    static constexpr int STRIDE = 1 << ((LG - 1) / 2);     // 2^floor(log2(nchan)/2)
+    static constexpr int NCH = CPLX ? F : F / 2;
+    static constexpr int STRIDE = 1 << ((CPLX ? LG : LG - 1) / 2);
     static constexpr int SUB = 2 * STRIDE;
     static constexpr int NSTEP = F / SUB;
     static constexpr int NFSTEP = NCH / STRIDE;
@@ -455,11 +457,11 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
 // The fused F-stage kernel: one block = one antenna, one polarisation, G
 // consecutive FFTs.
 
-template <int LG, int V, int PREC, bool TAP>
+template <int LG, int V, int PREC, bool TAP, bool CPLX = false>
 __global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_eu(Geo<LG, V>::WPE)))
 void fx_fstage_kernel(FStageArgs args)
 {
-    using Gm = Geo<LG, V>;
+    using Gm = Geo<LG, V, CPLX>;
     constexpr int F = Gm::F, NCH = Gm::NCH, NT = Gm::NT, FP = Gm::FP;
     constexpr int STRIDE = Gm::STRIDE, SUB = Gm::SUB, NSTEP = Gm::NSTEP, NFSTEP = Gm::NFSTEP;
     constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2, RL = Gm::RL, PL = Gm::PL;
@@ -467,13 +469,16 @@ void fx_fstage_kernel(FStageArgs args)
     constexpr int T0 = F / R0, PER0 = T0 / NT;
     constexpr int TL = F / RL, PERL = TL / NT;
     static_assert(PER0 * NT == T0 && PERL * NT == TL, "plan/thread mismatch");
-    constexpr int NOUT = PERL * (RL / 2);
+    constexpr int RO = CPLX ? RL : RL / 2;               // channels per last-pass butterfly
+    constexpr int NOUT = PERL * RO;
     static_assert(NOUT == Gm::NOUT, "output count");
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    __shared__ __align__(16) float lut[2][256];         // [time slot][byte] -> level of this pol
-    __shared__ __align__(16) uint32_t bytes[F / 8 + 4];
+    // real input: [time slot][byte] -> level of this pol, and the packed bytes
+    // of the FFT window; complex input reads its int8 pairs from global memory
+    __shared__ __align__(16) float lut[CPLX ? 1 : 2][256];
+    __shared__ __align__(16) uint32_t bytes[CPLX ? 4 : F / 8 + 4];
     __shared__ __align__(16) float2 tab[Gm::NTAB];       // rotator tables of this (FFT, antenna)
     __shared__ float2 tw1[NTW1];                         // exp(-2 pi i m / (R0 R1))
     const float2* rsub = tab;
@@ -481,7 +486,7 @@ void fx_fstage_kernel(FStageArgs args)
     const float2* frsub = tab + SUB + NSTEP;
     const float2* frstep = tab + SUB + NSTEP + STRIDE;
     constexpr int TPT = (Gm::NTAB + NT - 1) / NT;       // table entries per thread
-    constexpr int WPT = (F / 8 + 1 + NT - 1) / NT;       // window words per thread
+    constexpr int WPT = CPLX ? 1 : (F / 8 + 1 + NT - 1) / NT;  // window words per thread (real input)
 
     const FxPlan& p = args.p;
 
@@ -503,12 +508,14 @@ void fx_fstage_kernel(FStageArgs args)
     int tid = threadIdx.x;
 
     for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
-    for (int e = tid; e < 256; e += NT) {
-        // pol p of time slot t sits at bits 2*(2t+p) (kernel.c:195-200)
-        const float hi = 3.3359f;
-        const float lv[4] = {-hi, -1.0f, 1.0f, hi};
-        lut[0][e] = lv[(e >> (2 * pol)) & 3];
-        lut[1][e] = lv[(e >> (4 + 2 * pol)) & 3];
+    if constexpr (!CPLX) {
+        for (int e = tid; e < 256; e += NT) {
+            // pol p of time slot t sits at bits 2*(2t+p) (kernel.c:195-200)
+            const float hi = 3.3359f;
+            const float lv[4] = {-hi, -1.0f, 1.0f, hi};
+            lut[0][e] = lv[(e >> (2 * pol)) & 3];
+            lut[1][e] = lv[(e >> (4 + 2 * pol)) & 3];
+        }
     }
 
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
@@ -532,13 +539,15 @@ void fx_fstage_kernel(FStageArgs args)
             const int e = threadIdx.x + t * NT;
             f.t[t] = e < Gm::NTAB ? tb[e] : make_float2(0.0f, 0.0f);
         }
-        const int bb = (f.offset >> 1) & ~3;
-        const int nd = ((((f.offset + F - 1) >> 1) - bb) >> 2) + 1;
-        const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src + bb);
+        if constexpr (!CPLX) {
+            const int bb = (f.offset >> 1) & ~3;
+            const int nd = ((((f.offset + F - 1) >> 1) - bb) >> 2) + 1;
+            const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src + bb);
 #pragma unroll
-        for (int w = 0; w < WPT; w++) {
-            const int k = threadIdx.x + w * NT;
-            f.w[w] = k < nd ? s32[k] : 0u;
+            for (int w = 0; w < WPT; w++) {
+                const int k = threadIdx.x + w * NT;
+                f.w[w] = k < nd ? s32[k] : 0u;
+            }
         }
     };
     Prefetch cur;
@@ -560,7 +569,7 @@ void fx_fstage_kernel(FStageArgs args)
 #pragma unroll
                 for (int m = 0; m < PERL; m++)
 #pragma unroll
-                    for (int r = 0; r < RL / 2; r++)
+                    for (int r = 0; r < RO; r++)
                         args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + tid + m * NT + r * PL] =
                             make_float2(0.0f, 0.0f);
             }
@@ -575,16 +584,39 @@ void fx_fstage_kernel(FStageArgs args)
             const int e = tid + t * NT;
             if (e < Gm::NTAB) tab[e] = cur.t[t];
         }
+        if constexpr (!CPLX) {
 #pragma unroll
-        for (int w = 0; w < WPT; w++) {
-            const int k = tid + w * NT;
-            if (k < F / 8 + 4) bytes[k] = cur.w[w];
+            for (int w = 0; w < WPT; w++) {
+                const int k = tid + w * NT;
+                if (k < F / 8 + 4) bytes[k] = cur.w[w];
+            }
         }
         __syncthreads();
         if (gi + 1 < G) fetch(gi + 1, cur);
 
         // ---- pass 0: unpack + fringe rotate + radix-R0 butterfly
-        {
+        if constexpr (CPLX) {
+            // sample n = o + i + r*T0 is the word at byte 4n: this pol's int8
+            // (re, im) in its low (pol 0) or high (pol 1) half
+            const uint32_t* w32 = reinterpret_cast<const uint32_t*>(src) + o;
+#pragma unroll
+            for (int m = 0; m < PER0; m++) {
+                const int i = tid + m * NT;
+                const float2 rs = rsub[i & (SUB - 1)];
+                const float2* sp = rstep + i / SUB;
+                float2 x[R0];
+#pragma unroll
+                for (int r = 0; r < R0; r++) {
+                    const uint32_t h = w32[i + r * T0] >> (16 * pol);
+                    const float2 v = make_float2((float)(int8_t)(h & 0xffu), (float)(int8_t)((h >> 8) & 0xffu));
+                    x[r] = cmul(v, cmul(rs, sp[r * (T0 / SUB)]));
+                }
+                dft<R0>(x);
+                float2* wr = buf + pad(i * R0);
+#pragma unroll
+                for (int r = 0; r < R0; r++) wr[r] = x[r];
+            }
+        } else {
             const uint8_t* bl = reinterpret_cast<const uint8_t*>(bytes);
 #pragma unroll
             for (int m = 0; m < PER0; m++) {
@@ -617,7 +649,8 @@ void fx_fstage_kernel(FStageArgs args)
         mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
         if constexpr (Gm::NP == 4) mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
 
-        // ---- last pass: radix RL, channels < nchan only, frac-sample rotation
+        // ---- last pass: radix RL, the channels (real input: bins < nchan),
+        // frac-sample rotation
 #pragma unroll
         for (int m = 0; m < PERL; m++) {
             const int i = tid + m * NT;
@@ -626,15 +659,15 @@ void fx_fstage_kernel(FStageArgs args)
 #pragma unroll
             for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
             twiddle_pow<RL>(x, args.tw, i);
-            dft<RL, true>(x);
+            dft<RL, !CPLX>(x);
 #pragma unroll
-            for (int r = 0; r < RL / 2; r++) {
+            for (int r = 0; r < RO; r++) {
                 const int c = i + r * PL;
                 const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
                 const float2 y = cmul(fr, x[r]);
                 float si = 1.0f;
                 if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
-                outr.put_last(m * (RL / 2) + r, y, si);
+                outr.put_last(m * RO + r, y, si);
                 if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
             }
         }
@@ -646,9 +679,9 @@ void fx_fstage_kernel(FStageArgs args)
     for (int m = 0; m < PERL; m++) {
         const int i = tid + m * NT;
 #pragma unroll
-        for (int r = 0; r < RL / 2; r++) {
+        for (int r = 0; r < RO; r++) {
             const int c = i + r * PL;
-            const int ou = m * (RL / 2) + r;
+            const int ou = m * RO + r;
             const long long row = xrow0 + (long long)c * args.kc * p.Spad;
             if constexpr (PREC == FX_PREC_F16) {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
@@ -993,7 +1026,7 @@ hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int l
 
 bool fstage_supported(int lg) { return lg >= 9 && lg <= 15; }
 
-template <int LG, int V>
+template <int LG, int V, bool CPLX>
 static hipError_t launch_lgv(const FStageArgs& a, int prec, hipStream_t s)
 {
     const int nblocks = 2 * a.ngroups * a.p.nant;     // one block per polarisation
@@ -1002,16 +1035,16 @@ static hipError_t launch_lgv(const FStageArgs& a, int prec, hipStream_t s)
     const bool tap = a.tap != nullptr;
     switch (prec) {
     case FX_PREC_F32:
-        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F32, true><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, V, FX_PREC_F32, false><<<grid, block, 0, s>>>(a);
+        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F32, true, CPLX><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, V, FX_PREC_F32, false, CPLX><<<grid, block, 0, s>>>(a);
         break;
     case FX_PREC_F16:
-        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F16, true><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, V, FX_PREC_F16, false><<<grid, block, 0, s>>>(a);
+        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F16, true, CPLX><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, V, FX_PREC_F16, false, CPLX><<<grid, block, 0, s>>>(a);
         break;
     case FX_PREC_F8:
-        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F8, true><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, V, FX_PREC_F8, false><<<grid, block, 0, s>>>(a);
+        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F8, true, CPLX><<<grid, block, 0, s>>>(a);
+        else fx_fstage_kernel<LG, V, FX_PREC_F8, false, CPLX><<<grid, block, 0, s>>>(a);
         break;
     default: return hipErrorInvalidValue;
     }
@@ -1021,9 +1054,14 @@ static hipError_t launch_lgv(const FStageArgs& a, int prec, hipStream_t s)
 template <int LG>
 static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
 {
+    if (a.p.iscomplex) {
+        if constexpr (nplans(LG) > 1)
+            if (a.plan == 1) return launch_lgv<LG, 1, true>(a, prec, s);
+        return launch_lgv<LG, 0, true>(a, prec, s);
+    }
     if constexpr (nplans(LG) > 1)
-        if (a.plan == 1) return launch_lgv<LG, 1>(a, prec, s);
-    return launch_lgv<LG, 0>(a, prec, s);
+        if (a.plan == 1) return launch_lgv<LG, 1, false>(a, prec, s);
+    return launch_lgv<LG, 0, false>(a, prec, s);
 }
 
 int fstage_plans(int lg) { return lg == kEoLG ? nplans(kEoLG - 1) : nplans(lg); }
@@ -1038,9 +1076,13 @@ hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
     case 12: return launch_lg<12>(a, prec, s);
     case 13: return launch_lg<13>(a, prec, s);
     case 14: return launch_lg<14>(a, prec, s);
-    case 15: return a.plan == 1 ? launch_eo<1>(a, prec, s) : launch_eo<0>(a, prec, s);
+    case 15:
+        if (a.p.iscomplex) return hipErrorInvalidValue;
+        return a.plan == 1 ? launch_eo<1>(a, prec, s) : launch_eo<0>(a, prec, s);
     default: return hipErrorInvalidValue;
     }
 }
+
+bool fstage_supported_complex(int lg) { return lg >= 9 && lg <= 14; }
 
 }  // namespace fx

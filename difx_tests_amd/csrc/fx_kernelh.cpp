@@ -57,6 +57,7 @@ fx_context* session_ctx(int nant, int nchan, double lo, double bw, int numffts, 
     prm.prec = FX_PREC_F32;            // the reference path is FP32 throughout
     prm.max_ffts = numffts;
     prm.max_bytes_per_ant = bytes;
+    prm.iscomplex = 0;                 // the reference's CPU path takes real 2-bit data only
     if (fx_create(&prm, &g.ctx)) die_fx("GPU context");
     g.nant = nant;
     g.nchan = nchan;

@@ -42,6 +42,7 @@ hipError_t launch_rotators(const FxPlan& p, const FxDelay* dtab, int nffts, cons
                            const float* stepchf, float2* rtab, hipStream_t s);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
 bool fstage_supported(int lg);
+bool fstage_supported_complex(int lg);   // complex 8-bit input, F = nchan
 int fstage_plans(int lg);
 // Batched forward C2C FFT of length 2^lg (natural order in/out, unnormalised).
 hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int lg, int batch, hipStream_t s);

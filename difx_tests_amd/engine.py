@@ -50,7 +50,8 @@ class FxError(RuntimeError):
 
 class _Params(C.Structure):
     _fields_ = [("nant", C.c_int), ("nchan", C.c_int), ("lo", C.c_double), ("bw", C.c_double),
-                ("prec", C.c_int), ("max_ffts", C.c_int), ("max_bytes_per_ant", C.c_longlong)]
+                ("prec", C.c_int), ("max_ffts", C.c_int), ("max_bytes_per_ant", C.c_longlong),
+                ("iscomplex", C.c_int)]
 
 
 _lib = None
@@ -116,12 +117,15 @@ class FxEngine:
     """One subband on the GPU: delay model, packed input, F-stage, X-engine."""
 
     def __init__(self, nant: int, nchan: int, lo: float, bw: float, prec: int | str = PREC_F32,
-                 max_ffts: int = 64, max_bytes_per_ant: int = 0):
+                 max_ffts: int = 64, max_bytes_per_ant: int = 0, complex8: bool = False):
+        """complex8: complex 8-bit input (4 bytes per time sample: pol 0 re, im,
+        pol 1 re, im as int8), F = nchan; otherwise real 2-bit, F = 2*nchan."""
         self.lib = load_library()
         if isinstance(prec, str):
             prec = PREC_NAMES[prec]
         self.nant, self.nchan, self.lo, self.bw, self.prec = nant, nchan, float(lo), float(bw), prec
-        prm = _Params(nant, nchan, float(lo), float(bw), prec, max_ffts, max_bytes_per_ant)
+        self.complex8 = bool(complex8)
+        prm = _Params(nant, nchan, float(lo), float(bw), prec, max_ffts, max_bytes_per_ant, 1 if complex8 else 0)
         h = C.c_void_p()
         self._check(self.lib.fx_create(C.byref(prm), C.byref(h)))
         self.h = h

@@ -68,3 +68,36 @@ def make_data(nant: int, nsamp: int, seed: int, delays_samples=None, rho_common:
 def random_bytes(nant: int, nbytes: int, seed: int) -> np.ndarray:
     """Uniform random packed bytes (every code equally likely)."""
     return _rng(seed, 3, 0, 0).integers(0, 256, size=(nant, nbytes), dtype=np.uint8)
+
+
+# ---- complex 8-bit samples (COMPLEX 1, NBIT 8): one time sample is 4 bytes,
+# pol 0 (re, im) then pol 1 (re, im) as int8 (gkernel.cu:279-287)
+
+COMPLEX8_SCALE = 24.0        # int8 units per unit rms of each component
+
+
+def make_data_complex8(nant: int, nsamp: int, seed: int, delays_samples=None, rho_common: float = 1.0 / 3.0,
+                       identical: bool = False) -> np.ndarray:
+    """[nant][4*nsamp] bytes (int8 stored as uint8) of complex Gaussian
+    samples with the common-signal model of make_data, rounded and clipped
+    to int8 at COMPLEX8_SCALE per unit rms."""
+    d = np.zeros(nant, np.int64) if delays_samples is None else np.asarray(delays_samples, np.int64)
+    margin = int(np.abs(d).max()) if nant else 0
+    out = np.empty((nant, 4 * nsamp), np.uint8)
+    common = [_rng(seed, 11, 0, p).standard_normal((nsamp + 2 * margin, 2), dtype=np.float32) for p in range(2)]
+    scale = np.float32(COMPLEX8_SCALE / SIGMA)
+    for j in range(nant):
+        v = np.empty((nsamp, 4), np.float32)
+        for p in range(2):
+            s = common[p][margin + d[j]: margin + d[j] + nsamp]
+            key = 0 if identical else j
+            noise = _rng(seed, 12, key, p).standard_normal((nsamp, 2), dtype=np.float32)
+            v[:, 2 * p: 2 * p + 2] = s * np.float32(rho_common) + noise
+        q = np.clip(np.rint(v * scale), -128, 127).astype(np.int8)
+        out[j] = q.reshape(-1).view(np.uint8)
+    return out
+
+
+def random_bytes_complex8(nant: int, nsamp: int, seed: int) -> np.ndarray:
+    """Uniform random int8 components (as uint8), 4 bytes per time sample."""
+    return _rng(seed, 13, 0, 0).integers(0, 256, size=(nant, 4 * nsamp), dtype=np.uint8)

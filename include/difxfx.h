@@ -43,6 +43,12 @@ typedef struct {
     int prec;                    /* FX_PREC_*                                      */
     int max_ffts;                /* FFT slots held for the X-engine                */
     long long max_bytes_per_ant; /* packed input capacity per antenna              */
+    int iscomplex;               /* 0: real 2-bit samples, F = 2*nchan (the
+                                    reference's path); 1: complex 8-bit samples,
+                                    F = nchan, Ts = 1/bw, every FFT bin a channel
+                                    (band taken as [0, bw)); a time sample is 4
+                                    bytes: pol 0 re, im, pol 1 re, im as int8
+                                    (gkernel.cu:279-287)                          */
 } fx_params;
 
 /* Status codes */

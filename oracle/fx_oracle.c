@@ -43,8 +43,36 @@ int or_plan_init(or_plan *p, int nchan, double lo, double bw)
     p->lo = lo;
     p->bw = bw;
     p->fracLO = (lo - (double)(int)lo) > OR_TINY; /* kernel.c:39 */
+    p->iscomplex = 0;
     p->origin = 0;
     return 0;
+}
+
+int or_plan_init_complex(or_plan *p, int nchan, double lo, double bw)
+{
+    int rc = or_plan_init(p, nchan, lo, bw);
+    if (rc) return rc;
+    p->F = nchan;                          /* cfactor 1 (allocDataHost.c:12-19) */
+    p->Ts = 1.0 / bw;                      /* no halving for complex sampling */
+    p->nstep = p->F / p->sub;
+    p->iscomplex = 1;
+    return 0;
+}
+
+long long or_subint_bytes_complex(int nchan, int numffts)
+{
+    return (long long)nchan * (numffts + 1) * 4;
+}
+
+void or_unpack8complex(const uint8_t *data, int offset, int F, or_cf32 *x0, or_cf32 *x1)
+{
+    const int8_t *d = (const int8_t *)data + 4 * (size_t)offset;
+    for (int i = 0; i < F; i++) {
+        x0[i].re = (float)d[4 * i];
+        x0[i].im = (float)d[4 * i + 1];
+        x1[i].re = (float)d[4 * i + 2];
+        x1[i].im = (float)d[4 * i + 3];
+    }
 }
 
 long long or_subint_bytes(int nchan, int numffts)
@@ -314,6 +342,7 @@ void or_fft_f64(int F, const or_cf64 *in, or_cf64 *out)
 
 typedef struct {
     float *l0, *l1;         /* unpacked levels, F each */
+    or_cf32 *x0, *x1;       /* unpacked complex samples, F each */
     or_cf32 *rot, *u, *y;   /* rotator, rotated input, FFT output (F each) */
     or_cf32 *fr;            /* frac rotator (nchan) */
 } f32_scratch;
@@ -322,16 +351,18 @@ static int scratch_alloc(f32_scratch *s, int F, int nchan)
 {
     s->l0 = (float *)malloc(sizeof(float) * (size_t)F);
     s->l1 = (float *)malloc(sizeof(float) * (size_t)F);
+    s->x0 = (or_cf32 *)malloc(sizeof(or_cf32) * (size_t)F);
+    s->x1 = (or_cf32 *)malloc(sizeof(or_cf32) * (size_t)F);
     s->rot = (or_cf32 *)malloc(sizeof(or_cf32) * (size_t)F);
     s->u = (or_cf32 *)malloc(sizeof(or_cf32) * (size_t)F);
     s->y = (or_cf32 *)malloc(sizeof(or_cf32) * (size_t)F);
     s->fr = (or_cf32 *)malloc(sizeof(or_cf32) * (size_t)nchan);
-    return (s->l0 && s->l1 && s->rot && s->u && s->y && s->fr) ? 0 : -1;
+    return (s->l0 && s->l1 && s->x0 && s->x1 && s->rot && s->u && s->y && s->fr) ? 0 : -1;
 }
 
 static void scratch_free(f32_scratch *s)
 {
-    free(s->l0); free(s->l1); free(s->rot); free(s->u); free(s->y); free(s->fr);
+    free(s->l0); free(s->l1); free(s->x0); free(s->x1); free(s->rot); free(s->u); free(s->y); free(s->fr);
 }
 
 /* Computes the two frac-corrected spectra (channels < nchan) of antenna j at
@@ -348,14 +379,22 @@ static int fstage_one_f32(const or_plan *p, const double *c, double fileoffset, 
     if (offset_out) *offset_out = offset;
     if (fd_out) *fd_out = fd;
     if (!valid) return 0;
-    or_unpack2bit(data, offset, F, s->l0, s->l1);
     or_fringe_rotator_f32(p, a, b, s->rot);
     or_frac_rotator_f32(p, fd, s->fr);
+    if (p->iscomplex)
+        or_unpack8complex(data, offset, F, s->x0, s->x1);
+    else
+        or_unpack2bit(data, offset, F, s->l0, s->l1);
     for (int pol = 0; pol < 2; pol++) {
-        const float *l = pol ? s->l1 : s->l0;
-        for (int i = 0; i < F; i++) {
-            s->u[i].re = l[i] * s->rot[i].re;
-            s->u[i].im = l[i] * s->rot[i].im;
+        if (p->iscomplex) {
+            const or_cf32 *x = pol ? s->x1 : s->x0;
+            for (int i = 0; i < F; i++) s->u[i] = cmul32(x[i], s->rot[i]);
+        } else {
+            const float *l = pol ? s->l1 : s->l0;
+            for (int i = 0; i < F; i++) {
+                s->u[i].re = l[i] * s->rot[i].re;
+                s->u[i].im = l[i] * s->rot[i].im;
+            }
         }
         or_fft_f32(F, s->u, s->y);
         for (int ch = 0; ch < nchan; ch++) out[pol * nchan + ch] = cmul32(s->fr[ch], s->y[ch]);
@@ -584,14 +623,20 @@ int or_process_f64(const or_plan *p, int nant, int numffts, int split,
                 if (valid_out) valid_out[(size_t)k * nant + j] = valid[j];
                 or_cf64 *sj = spec + (size_t)j * 2 * nchan;
                 if (valid[j]) {
-                    or_unpack2bit(data[j], off, F, tl0, tl1);
                     or_fringe_rotator_f64(p, a, b, trot);
                     or_frac_rotator_f64(p, fd, tfr);
+                    const int8_t *cd = (const int8_t *)data[j] + 4 * (size_t)off;
+                    if (!p->iscomplex) or_unpack2bit(data[j], off, F, tl0, tl1);
                     for (int pol = 0; pol < 2; pol++) {
                         const float *l = pol ? tl1 : tl0;
                         for (int i = 0; i < F; i++) {
-                            tu[i].re = (double)l[i] * trot[i].re;
-                            tu[i].im = (double)l[i] * trot[i].im;
+                            if (p->iscomplex) {
+                                const or_cf64 x = {(double)cd[4 * i + 2 * pol], (double)cd[4 * i + 2 * pol + 1]};
+                                tu[i] = cmul64(x, trot[i]);
+                            } else {
+                                tu[i].re = (double)l[i] * trot[i].re;
+                                tu[i].im = (double)l[i] * trot[i].im;
+                            }
                         }
                         or_fft_f64(F, tu, ty);
                         for (int ch = 0; ch < nchan; ch++) sj[pol * nchan + ch] = cmul64(tfr[ch], ty[ch]);

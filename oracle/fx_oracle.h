@@ -53,6 +53,8 @@ typedef struct {
     double lo;      /* LO frequency (Hz)                                    */
     double bw;      /* bandwidth (Hz)                                       */
     int fracLO;     /* (lo - (int)lo) > 1e-11                               */
+    int iscomplex;  /* 0: real 2-bit samples (the reference's CPU path);
+                       1: complex 8-bit samples, F = nchan, Ts = 1/bw        */
     int origin;     /* global FFT index of the first FFT of the data buffer:
                        o = (g - origin)*F - n.  0 (set by or_plan_init) is the
                        reference, whose buffer starts at FFT 0; a streaming
@@ -61,6 +63,23 @@ typedef struct {
 
 /* Returns 0 on success, -1 if nchan is not a power of two >= 4. */
 int or_plan_init(or_plan *p, int nchan, double lo, double bw);
+
+/* Complex 8-bit input (the COMPLEX keyword, allocDataHost.c:12-19; the
+ * reference's CPU path rejects it, kernel.c:19-28, its GPU path unpacks it,
+ * gkernel.cu:277-296).  The same semantics as the real path with F = nchan
+ * complex samples per FFT, Ts = 1/bw, every FFT bin a channel (the band is
+ * taken as [0, bw)), stride/sub/nstep/nfstep from nchan by the same rules. */
+int or_plan_init_complex(or_plan *p, int nchan, double lo, double bw);
+
+/* Bytes per antenna of a complex 8-bit dual-polarisation subint: numffts + 1
+ * FFTs of nchan samples, 4 bytes each (pol 0 re, im, pol 1 re, im as int8;
+ * gkernel.cu:279-287).  allocDataHost.c:21 gives half of this for nbit 8
+ * complex (it counts one byte per complex sample and polarisation). */
+long long or_subint_bytes_complex(int nchan, int numffts);
+
+/* The F complex samples of both polarisations starting at sample `offset`
+ * (byte 4*offset): x0 = pol 0, x1 = pol 1. */
+void or_unpack8complex(const uint8_t *data, int offset, int F, or_cf32 *x0, or_cf32 *x1);
 
 /* kernel.c:373-386 getStationDelay, FP64, same evaluation order. */
 void or_station_delay(const double c[3], int fftindex, double *mean, double *a, double *b);

@@ -18,7 +18,7 @@ SO = os.path.join(HERE, "_build", "libfxoracle.so")
 class OrPlan(C.Structure):
     _fields_ = [("nchan", C.c_int), ("F", C.c_int), ("stride", C.c_int), ("sub", C.c_int), ("nstep", C.c_int),
                 ("nfstep", C.c_int), ("Ts", C.c_double), ("lo", C.c_double), ("bw", C.c_double),
-                ("fracLO", C.c_int), ("origin", C.c_int)]
+                ("fracLO", C.c_int), ("iscomplex", C.c_int), ("origin", C.c_int)]
 
 
 _lib = None
@@ -41,6 +41,10 @@ def lib() -> C.CDLL:
         I = C.c_int
         D = C.c_double
         L.or_plan_init.argtypes = [C.POINTER(OrPlan), I, D, D]
+        L.or_plan_init_complex.argtypes = [C.POINTER(OrPlan), I, D, D]
+        L.or_subint_bytes_complex.argtypes = [I, I]
+        L.or_subint_bytes_complex.restype = C.c_longlong
+        L.or_unpack8complex.argtypes = [P, I, I, P, P]
         L.or_station_delay.argtypes = [P, I, P, P, P]
         L.or_offset.argtypes = [D, D, D, I, I, I, P, P]
         L.or_offset_origin.argtypes = [D, D, D, I, I, I, I, P, P]
@@ -68,10 +72,11 @@ def _p(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
-def plan(nchan: int, lo: float, bw: float, origin: int = 0) -> OrPlan:
-    """origin: global FFT index of the data buffer's first FFT (0: the reference)."""
+def plan(nchan: int, lo: float, bw: float, origin: int = 0, complex8: bool = False) -> OrPlan:
+    """origin: global FFT index of the data buffer's first FFT (0: the reference);
+    complex8: 8-bit complex input (F = nchan, Ts = 1/bw)."""
     p = OrPlan()
-    rc = lib().or_plan_init(C.byref(p), nchan, lo, bw)
+    rc = (lib().or_plan_init_complex if complex8 else lib().or_plan_init)(C.byref(p), nchan, lo, bw)
     if rc:
         raise ValueError(f"or_plan_init failed ({rc})")
     p.origin = origin
@@ -171,6 +176,19 @@ def offset(mean, fileoff, Ts, g, F, maxoffset, origin=None):
     else:
         v = lib().or_offset_origin(mean, fileoff, Ts, g, origin, F, maxoffset, C.byref(o), C.byref(fd))
     return v, o.value, fd.value
+
+
+def unpack8complex(data_row, offset_, F):
+    """(pol 0, pol 1) complex64 samples [F] starting at sample offset_."""
+    a = np.zeros(F, np.complex64)
+    b = np.zeros(F, np.complex64)
+    d = np.ascontiguousarray(data_row, np.uint8)
+    lib().or_unpack8complex(_p(d), offset_, F, _p(a), _p(b))
+    return a, b
+
+
+def subint_bytes_complex(nchan, numffts):
+    return int(lib().or_subint_bytes_complex(nchan, numffts))
 
 
 def unpack(data_row, offset_, F):

@@ -380,3 +380,61 @@ def test_streaming_subints_equal_whole_stream_in_the_interior():
                     assert np.array_equal(part["chan"][k, j], whole["chan"][g, j])
                     compared += 1
     assert compared >= S * nf * nant - 2 * S * nant
+
+
+# ---- complex 8-bit input
+
+def test_complex_plan_geometry():
+    p = orc.plan(1024, 1.65e9, 32e6, complex8=True)
+    assert (p.nchan, p.F, p.stride, p.sub, p.nstep, p.nfstep, p.iscomplex) == (1024, 1024, 32, 64, 16, 32, 1)
+    assert p.Ts == 1.0 / 32e6
+    r = orc.plan(1024, 1.65e9, 32e6)
+    assert (r.F, r.nstep, r.iscomplex, r.Ts) == (2048, 32, 0, 1.0 / 64e6)
+    assert orc.subint_bytes_complex(1024, 10) == 1024 * 11 * 4
+
+
+def test_unpack8complex_brute_force():
+    rng = np.random.default_rng(21)
+    raw = rng.integers(0, 256, 4 * 300, dtype=np.uint8)
+    x0, x1 = orc.unpack8complex(raw, 17, 200)
+    v = raw.view(np.int8).reshape(300, 4).astype(np.float32)
+    assert np.array_equal(x0, (v[17:217, 0] + 1j * v[17:217, 1]).astype(np.complex64))
+    assert np.array_equal(x1, (v[17:217, 2] + 1j * v[17:217, 3]).astype(np.complex64))
+
+
+def test_complex_fstage_matches_numpy_composition():
+    # independent composition: int8 pairs -> x * rotator -> numpy FFT -> every
+    # bin times the fractional rotator
+    nchan, nant, nf = 256, 2, 3
+    p = orc.plan(nchan, 1.65e9, 16e6, complex8=True)
+    rng = np.random.default_rng(22)
+    delays = np.stack([np.zeros(nant), rng.uniform(-1e-10, 1e-10, nant), rng.uniform(-1e-6, 1e-6, nant)], axis=1)
+    fileoff = delays[:, 2] + rng.uniform(-0.4, 0.4, nant) * p.Ts
+    data = synth.random_bytes_complex8(nant, (nf + 1) * nchan, 22)
+    it = np.arange(nf, dtype=np.int32)
+    ref = orc.process_f32(p, data, delays, fileoff, it, want_chan=True)
+    ref64 = orc.process_f64(p, data, delays, fileoff, it, want_chan=True)
+    for k in range(nf):
+        for j in range(nant):
+            m, a, b = orc.station_delay(delays[j], k)
+            v, o, fd = orc.offset(m, fileoff[j], p.Ts, k, p.F, (nf - 1) * p.F)
+            assert v == ref["valid"][k, j] and o == ref["offset"][k, j]
+            x0, x1 = orc.unpack8complex(data[j], o, p.F)
+            rot = orc.fringe_rotator(p, a, b, f64=True)
+            fr = orc.frac_rotator(p, fd, f64=True)
+            for pol, x in enumerate((x0, x1)):
+                want = fr * np.fft.fft(x.astype(np.complex128) * rot)
+                got = ref["chan"][k, j, pol]
+                assert np.linalg.norm(got - want) / np.linalg.norm(want) < 2e-6
+                assert np.linalg.norm(ref64["chan"][k, j, pol] - want) / np.linalg.norm(want) < 1e-12
+
+
+def test_complex_identical_antennas_cross_equals_auto():
+    nchan, nf = 128, 4
+    p = orc.plan(nchan, 1.65e9, 16e6, complex8=True)
+    delays = np.tile([0.0, 0.0, 3e-7], (2, 1))
+    fileoff = np.full(2, 3e-7)
+    data = synth.make_data_complex8(2, (nf + 1) * nchan, 23, identical=True)
+    r = orc.process_f64(p, data, delays, fileoff, np.arange(nf, dtype=np.int32))
+    assert r["count"][0] == nf
+    assert np.allclose(r["vis"][0], r["autos"][0], rtol=1e-12, atol=1e-9)
