@@ -3,6 +3,7 @@
 Outputs (git-ignored, shipped with the tree):
   difx_tests_amd/lib/libdifxfx.so   HIP kernels + native C API + kernel.h ABI
   difx_tests_amd/lib/fxmain         drop-in host program (reference main.c)
+  difx_tests_amd/lib/fxbatch        uneven-subband host program (native API)
   oracle/_build/libfxoracle.so      CPU oracle (test infrastructure only)
 
 Usage: python -m difx_tests_amd.build [--no-oracle]
@@ -75,6 +76,14 @@ def build_host() -> str:
     return exe
 
 
+def build_batch_host() -> str:
+    exe = os.path.join(LIBDIR, "fxbatch")
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", f"-I{INCLUDE}",
+          os.path.join(PKG, "host", "fxbatch.c"), "-o", exe,
+          f"-L{LIBDIR}", "-ldifxfx", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-lm"])
+    return exe
+
+
 def build_oracle() -> str:
     outdir = os.path.join(ORACLE, "_build")
     os.makedirs(outdir, exist_ok=True)
@@ -106,6 +115,7 @@ def build_all(oracle: bool = True) -> None:
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     build_library()
     build_host()
+    build_batch_host()
     if oracle:
         build_oracle()
         build_test_drivers()

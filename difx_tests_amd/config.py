@@ -6,6 +6,15 @@ lines `NAME FILE c0 c1 c2 OFFSET` (delay polynomial in seconds with t in FFT
 units, file start offset in seconds; loadConfig.h:17-31).  Defaults NPOL=2,
 COMPLEX=0, NBIT=2.  An empty line is an error (main.c:37-40); an unknown
 keyword only warns (main.c:123-125).
+
+Extensions read by the uneven-subband host program (fxbatch), which the
+reference would warn about and skip:
+  SUBBAND nchan bandwidth lo numffts antennas   (antennas: comma-separated
+                                                 indices into the NANT list,
+                                                 or ALL)
+  NSUBINT n                                     (subints to process)
+  PREC F32|F16|F8|F32V                          (X-engine precision)
+and an antenna FILE may hold one %d, replaced by the subband index.
 """
 from __future__ import annotations
 
@@ -26,6 +35,15 @@ class AntennaEntry:
 
 
 @dataclass
+class SubbandEntry:
+    nchan: int
+    bandwidth: float
+    lo: float
+    numffts: int
+    antennas: list[int] | None = None        # None: every antenna
+
+
+@dataclass
 class Config:
     nbit: int = 2
     npol: int = 2
@@ -35,6 +53,9 @@ class Config:
     bandwidth: float = 0.0
     numffts: int = 0
     antennas: list[AntennaEntry] = field(default_factory=list)
+    subbands: list[SubbandEntry] = field(default_factory=list)
+    nsubint: int = 1
+    prec: str = "F32"
 
     @property
     def nant(self) -> int:
@@ -83,6 +104,22 @@ def parse_config_text(text: str, warn=sys.stderr) -> Config:
             ant_to_read -= 1
             continue
         key = toks[0].upper()
+        if key == "SUBBAND":
+            if len(toks) < 6:
+                raise ConfigError("Insufficient data for SUBBAND")
+            ants = None if toks[5].upper() == "ALL" else [_c_int(t) for t in toks[5].split(",") if t]
+            cfg.subbands.append(SubbandEntry(_c_int(toks[1]), float(toks[2]), float(toks[3]), _c_int(toks[4]), ants))
+            continue
+        if key == "NSUBINT":
+            if len(toks) < 2:
+                raise ConfigError("Insufficient data for NSUBINT")
+            cfg.nsubint = _c_int(toks[1])
+            continue
+        if key == "PREC":
+            if len(toks) < 2 or toks[1].upper() not in ("F32", "F16", "F8", "F32V"):
+                raise ConfigError("PREC must be F32, F16, F8 or F32V")
+            cfg.prec = toks[1].upper()
+            continue
         if key in _INT_KEYS or key in _FLOAT_KEYS or key == "NANT":
             if len(toks) < 2:
                 raise ConfigError(f"Insufficient data for {key}")
@@ -115,4 +152,11 @@ def format_config(cfg: Config) -> str:
              f"NANT {cfg.nant}"]
     for a in cfg.antennas:
         lines.append(f"{a.name} {a.file} {a.delay[0]:.17g} {a.delay[1]:.17g} {a.delay[2]:.17g} {a.fileoffset:.17g}")
+    for sb in cfg.subbands:
+        ants = "ALL" if sb.antennas is None else ",".join(str(a) for a in sb.antennas)
+        lines.append(f"SUBBAND {sb.nchan} {sb.bandwidth:.17g} {sb.lo:.17g} {sb.numffts} {ants}")
+    if cfg.subbands or cfg.nsubint != 1:
+        lines.append(f"NSUBINT {cfg.nsubint}")
+    if cfg.prec != "F32":
+        lines.append(f"PREC {cfg.prec}")
     return "\n".join(lines) + "\n"

@@ -80,6 +80,7 @@ struct fx_context {
     float2* d_tap;
     int tap;
     int last_nffts;     // entries in the delay table of the last fx_fstage
+    int h_origin;       // host copy of the buffer origin being uploaded
     int xsb;            // X-engine super-block size (0: by the signal count)
     int fplan;          // F-stage radix plan
     int xbatch;         // subints per X-engine launch recommended to callers (1 unless tuned)
@@ -214,7 +215,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     }
     ALLOC(c->d_data, (size_t)c->stride * p.nant);
     ALLOC(c->d_model, sizeof(double) * 4 * p.nant);
-    ALLOC(c->d_iter, sizeof(int) * c->max_ffts);
+    ALLOC(c->d_iter, sizeof(int) * (c->max_ffts + 1));   // FFT indices, then the buffer origin
     ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * (p.sub + p.nstep + p.stride + p.nfstep));
     ALLOC(c->d_tw, sizeof(float2) * p.F);
@@ -405,6 +406,16 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
     return FX_OK;
 }
 
+// FFT indices and the buffer origin to the device (ahead of the launches
+// that read them, on stream s).
+static int iter_upload(fx_context* c, const int* iteration, const FStageCall& f, hipStream_t s)
+{
+    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * f.numffts, hipMemcpyHostToDevice, s));
+    c->h_origin = f.origin;
+    HIPCHK(hipMemcpyAsync(c->d_iter + c->max_ffts, &c->h_origin, sizeof(int), hipMemcpyHostToDevice, s));
+    return FX_OK;
+}
+
 // Delay table, F-stage and counts on stream s, reading c->d_iter.
 static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
 {
@@ -414,7 +425,7 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     const bool ingest = c->ingest && !c->capturing;
     if (ingest) HIPCHK(hipStreamWaitEvent(s, c->copied[f.slot], 0));
     TIMED(TM_DELAY, s, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, f.numffts, f.npad,
-                                    f.maxoffset, f.origin, c->d_dtab, s));
+                                    f.maxoffset, c->d_iter + c->max_ffts, c->d_dtab, s));
     TIMED(TM_DELAY, s, launch_rotators(p, c->d_dtab, f.npad, c->d_subchf, c->d_stepchf, c->d_rtab, s));
     FStageArgs a;
     a.p = p;
@@ -445,7 +456,8 @@ extern "C" int fx_fstage(fx_context* c, int numffts, const int* iteration, int s
     FStageCall f;
     int rc = fstage_check(c, numffts, iteration, split, fft0, &f);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+    rc = iter_upload(c, iteration, f, c->stream);
+    if (rc) return rc;
     rc = fstage_enqueue(c, f, c->stream);
     if (rc) return rc;
     if (nslots) *nslots = f.npad;
@@ -458,7 +470,8 @@ extern "C" int fx_fstage_ex(fx_context* c, int slot, int numffts, const int* ite
     FStageCall f;
     int rc = fstage_check(c, numffts, iteration, split, fft0, &f, origin, slot);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+    rc = iter_upload(c, iteration, f, c->stream);
+    if (rc) return rc;
     rc = fstage_enqueue(c, f, c->stream);
     if (rc) return rc;
     if (nslots) *nslots = f.npad;
@@ -912,7 +925,8 @@ static int batch_capture(fx_batch* b, const std::vector<FStageCall>& calls)
     return FX_OK;
 }
 
-extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* const* iteration, int split, int mode)
+extern "C" int fx_batch_process_ex(fx_batch* b, const int* numffts, const int* const* iteration,
+                                   const int* origins, int split, int mode)
 {
     if (!b || !numffts || !iteration) return fail(FX_EINVAL, "null argument");
     if (mode != FX_BATCH_GRAPH && mode != FX_BATCH_LAUNCH) return fail(FX_EINVAL, "bad mode %d", mode);
@@ -921,7 +935,7 @@ extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* cons
     std::vector<long long> key;
     key.reserve(2 * n + 1);
     for (int i = 0; i < n; i++) {
-        int rc = fstage_check(b->ctx[i], numffts[i], iteration[i], split, 0, &calls[i]);
+        int rc = fstage_check(b->ctx[i], numffts[i], iteration[i], split, 0, &calls[i], origins ? origins[i] : 0);
         if (rc) return fail(rc, "subband %d: %s", i, fx_last_error());
         key.push_back(numffts[i]);
         key.push_back(((long long)b->ctx[i]->tap * 8 + b->ctx[i]->xsb) * 8 + b->ctx[i]->fplan);
@@ -932,9 +946,10 @@ extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* cons
         HIPCHK(hipEventRecord(b->pre[i], b->ctx[i]->stream));
         HIPCHK(hipStreamWaitEvent(b->s0, b->pre[i], 0));
     }
-    for (int i = 0; i < n; i++)
-        HIPCHK(hipMemcpyAsync(b->ctx[i]->d_iter, iteration[i], sizeof(int) * numffts[i], hipMemcpyHostToDevice,
-                              b->s0));
+    for (int i = 0; i < n; i++) {
+        int rc = iter_upload(b->ctx[i], iteration[i], calls[i], b->s0);
+        if (rc) return rc;
+    }
     if (mode == FX_BATCH_GRAPH) {
         if (!b->exec || key != b->key) {
             int rc = batch_capture(b, calls);
@@ -956,6 +971,11 @@ extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* cons
         b->ctx[i]->last_nffts = calls[i].npad;
     }
     return FX_OK;
+}
+
+extern "C" int fx_batch_process(fx_batch* b, const int* numffts, const int* const* iteration, int split, int mode)
+{
+    return fx_batch_process_ex(b, numffts, iteration, nullptr, split, mode);
 }
 
 extern "C" int fx_batch_sync(fx_batch* b)
@@ -1082,7 +1102,8 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
     const int saved_timing = c->timing, saved_tap = c->tap;
     c->timing = 0;
     c->tap = 0;
-    HIPCHK(hipMemcpyAsync(c->d_iter, it.data(), sizeof(int) * numffts, hipMemcpyHostToDevice, c->stream));
+    rc = iter_upload(c, it.data(), f, c->stream);
+    if (rc) return rc;
 
     // F-stage radix plans
     double best_f = 1e30;

@@ -28,7 +28,8 @@ static constexpr double kPi = 3.14159265358979323846;
 
 __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
                                 const double* __restrict__ fileoff, const int* __restrict__ iteration,
-                                int numffts, int nffts_pad, int maxoffset, int origin, FxDelay* __restrict__ out)
+                                int numffts, int nffts_pad, int maxoffset, const int* __restrict__ origin_dev,
+                                FxDelay* __restrict__ out)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -42,6 +43,7 @@ __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
         return;
     }
     const int fftindex = iteration[k];
+    const int origin = *origin_dev;
     const double c0 = delays[3 * j], c1 = delays[3 * j + 1], c2 = delays[3 * j + 2];
     // kernel.c:378-385
     const double d0 = c0 * fftindex * fftindex + c1 * fftindex + c2;
@@ -73,12 +75,13 @@ __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
 }
 
 hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
-                        int numffts, int nffts_pad, int maxoffset, int origin, FxDelay* out, hipStream_t s)
+                        int numffts, int nffts_pad, int maxoffset, const int* origin_dev, FxDelay* out,
+                        hipStream_t s)
 {
     const int n = nffts_pad * p.nant;
     if (n <= 0) return hipSuccess;
     fx_delay_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, delays, fileoff, iteration, numffts, nffts_pad,
-                                                    maxoffset, origin, out);
+                                                    maxoffset, origin_dev, out);
     return hipGetLastError();
 }
 

@@ -36,8 +36,11 @@ struct XArgs {
 
 // F-stage: LG = log2(F); prec = FX_PREC_*; G is fixed per precision.
 int fx_fstage_group_size(int prec);
+// origin_dev: device int, global FFT index of the data buffer's first FFT
+// (read by the kernel, so a recorded graph serves every subint)
 hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
-                        int numffts, int nffts_pad, int maxoffset, int origin, FxDelay* out, hipStream_t s);
+                        int numffts, int nffts_pad, int maxoffset, const int* origin_dev, FxDelay* out,
+                        hipStream_t s);
 hipError_t launch_rotators(const FxPlan& p, const FxDelay* dtab, int nffts, const float* subchf,
                            const float* stepchf, float2* rtab, hipStream_t s);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);

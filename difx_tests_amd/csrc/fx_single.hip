@@ -319,12 +319,13 @@ void stationDelayAndOffset(double** delays, int* iteration, double* antFileOffse
     p.nant = 1;
     p.F = *fftchannels;
     p.Ts = *sampletime;
-    DBuf model(4 * sizeof(double)), it(sizeof(int)), tab(sizeof(FxDelay));
+    DBuf model(4 * sizeof(double)), it(2 * sizeof(int)), tab(sizeof(FxDelay));
     double m[4] = {delays[0][0], delays[0][1], delays[0][2], antFileOffsets[0]};
     h2d(model.p, m, sizeof(m));
-    h2d(it.p, iteration, sizeof(int));
+    const int io[2] = {*iteration, 0};                    // FFT index, buffer origin 0
+    h2d(it.p, io, sizeof(io));
     CK(fx::launch_delay(p, model.as<double>(), model.as<double>() + 3, it.as<int>(), 1, 1,
-                        (numffts - 1) * *fftchannels, 0, tab.as<FxDelay>(), 0));
+                        (numffts - 1) * *fftchannels, it.as<int>() + 1, tab.as<FxDelay>(), 0));
     FxDelay e;
     d2h(&e, tab.p, sizeof(e));
     *delaya = e.a;
@@ -349,16 +350,17 @@ void processAntennas(FxKernel* kernel, int* nant, int* numffts, int* fftchannels
     p.nant = *nant;
     p.F = F;
     p.Ts = *sampletime;
-    DBuf model(4 * sizeof(double) * *nant), it(sizeof(int)), tab(sizeof(FxDelay) * *nant);
+    DBuf model(4 * sizeof(double) * *nant), it(2 * sizeof(int)), tab(sizeof(FxDelay) * *nant);
     std::vector<double> m(4 * (size_t)*nant);
     for (int j = 0; j < *nant; j++) {
         for (int q = 0; q < 3; q++) m[3 * (size_t)j + q] = delays[j][q];
         m[3 * (size_t)*nant + j] = antFileOffsets[j];
     }
     h2d(model.p, m.data(), sizeof(double) * m.size());
-    h2d(it.p, &i, sizeof(int));
+    const int io[2] = {i, 0};                             // FFT index, buffer origin 0
+    h2d(it.p, io, sizeof(io));
     CK(fx::launch_delay(p, model.as<double>(), model.as<double>() + 3 * *nant, it.as<int>(), 1, 1,
-                        (*numffts - 1) * F, 0, tab.as<FxDelay>(), 0));
+                        (*numffts - 1) * F, it.as<int>() + 1, tab.as<FxDelay>(), 0));
     std::vector<FxDelay> e((size_t)*nant);
     d2h(e.data(), tab.p, sizeof(FxDelay) * *nant);
     for (int j = 0; j < *nant; j++) {

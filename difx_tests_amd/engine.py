@@ -25,7 +25,7 @@ NATIVE_SYMBOLS = [
     "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry", "fx_set_option", "fx_get_option",
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
     "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait", "fx_read_vis_ex",
-    "fx_save_vis",
+    "fx_save_vis", "fx_batch_process_ex",
 ]
 VIS_NORMALIZE = 1
 VIS_TEXT, VIS_BINARY = 0, 1
@@ -96,6 +96,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_batch_destroy.argtypes = [P]
     lib.fx_batch_destroy.restype = None
     lib.fx_batch_process.argtypes = [P, P, P, I, I]
+    lib.fx_batch_process_ex.argtypes = [P, P, P, P, I, I]
     lib.fx_batch_sync.argtypes = [P]
     lib.fx_batch_captures.argtypes = [P]
     lib.fx_autotune.argtypes = [P, I, I, C.c_char_p, P]
@@ -311,13 +312,15 @@ class FxBatch:
         if rc != 0:
             raise FxError(f"libdifxfx error {rc}: {self.lib.fx_last_error().decode()}")
 
-    def process(self, iterations, split: int = 1, mode: str = "graph") -> None:
+    def process(self, iterations, split: int = 1, mode: str = "graph", origins=None) -> None:
         its = [np.ascontiguousarray(i, dtype=np.int32) for i in iterations]
         assert len(its) == len(self.engines)
         n = np.array([len(i) for i in its], np.int32)
         ptrs = (C.c_void_p * len(its))(*[i.ctypes.data for i in its])
         m = BATCH_GRAPH if mode == "graph" else BATCH_LAUNCH
-        self._check(self.lib.fx_batch_process(self.h, _ptr(n), ptrs, split, m))
+        org = None if origins is None else np.ascontiguousarray(origins, dtype=np.int32)
+        self._check(self.lib.fx_batch_process_ex(self.h, _ptr(n), ptrs, None if org is None else _ptr(org), split,
+                                                 m))
 
     def sync(self) -> None:
         self._check(self.lib.fx_batch_sync(self.h))

@@ -212,6 +212,11 @@ typedef struct fx_batch fx_batch;
 int fx_batch_create(fx_context *const *ctxs, int n, fx_batch **out);
 void fx_batch_destroy(fx_batch *batch);
 int fx_batch_process(fx_batch *batch, const int *numffts, const int *const *iteration, int split, int mode);
+/* Same with a buffer origin per context (fx_fstage_ex); origins may be NULL
+ * (all 0).  Origins are read by the kernels from device memory, so one
+ * recorded graph serves every subint of a stream. */
+int fx_batch_process_ex(fx_batch *batch, const int *numffts, const int *const *iteration, const int *origins,
+                        int split, int mode);
 int fx_batch_sync(fx_batch *batch);
 /* Number of graph recordings so far (for tests). */
 int fx_batch_captures(const fx_batch *batch);

@@ -66,3 +66,15 @@ def test_engine_fails_loudly_without_library(tmp_path):
             engine.load_library(str(tmp_path / "missing.so"))
     finally:
         engine._lib = engine._lib_backup
+
+
+def test_subband_grammar_round_trip():
+    from difx_tests_amd import config as cfgmod
+    text = ("NBIT 2\nLO 1650000000\nNANT 3\nA a%d.bin 0 1e-12 1e-6 1e-6\nB b%d.bin 0 0 0 0\nC c%d.bin 0 0 2e-7 2e-7\n"
+            "SUBBAND 256 8000000 1650000000 40 0,2\nSUBBAND 1024 32000000 1700000000 10 ALL\nNSUBINT 3\nPREC f16\n")
+    cfg = cfgmod.parse_config_text(text, warn=None)
+    assert cfg.nant == 3 and cfg.nsubint == 3 and cfg.prec == "F16"
+    assert [(s.nchan, s.bandwidth, s.lo, s.numffts, s.antennas) for s in cfg.subbands] == \
+        [(256, 8e6, 1.65e9, 40, [0, 2]), (1024, 32e6, 1.7e9, 10, None)]
+    again = cfgmod.parse_config_text(cfgmod.format_config(cfg), warn=None)
+    assert again == cfg

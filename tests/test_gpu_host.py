@@ -163,3 +163,59 @@ def test_kernelh_fine_grained_functions(tmp_path, nchan):
     assert np.linalg.norm(vis[live] - ref["vis"][live]) / np.linalg.norm(ref["vis"][live]) < 1e-5
     want = ref["vis"][live] / cnt[live, None, None]
     assert np.linalg.norm(visn[live] - want) / np.linalg.norm(want) < 1e-5
+
+
+FXBATCH = os.path.join(ROOT, "difx_tests_amd", "lib", "fxbatch")
+
+
+def test_fxbatch_uneven_subbands(tmp_path):
+    # three subbands (channels, bandwidth, LO, antenna subset), two subints of
+    # a continuous stream per (subband, antenna) file; graph and plain
+    # launches; each subband's normalised visibilities against the oracle
+    from difx_tests_amd import visio
+    rng = np.random.default_rng(31)
+    nant, nsub = 5, 2
+    delays = np.stack([np.zeros(nant), rng.uniform(-2e-11, 2e-11, nant), rng.uniform(-1e-6, 1e-6, nant)], axis=1)
+    spread = rng.uniform(-0.45, 0.45, nant)
+    subbands = [(256, 8e6, 1.65e9, 12, [0, 1, 3]), (1024, 32e6, 1.70e9, 5, None), (512, 16e6, 1.6e9, 9, [4, 2, 0, 1])]
+    cfg = cfgmod.Config(nchan=256, lo=1.65e9, bandwidth=8e6, numffts=12, nsubint=nsub)
+    fileoff = delays[:, 2] + spread / (2 * 8e6)     # fixed seconds for every subband
+    for j in range(nant):
+        cfg.antennas.append(cfgmod.AntennaEntry(f"A{j}", f"a{j}_sb%d.bin", tuple(delays[j]), float(fileoff[j])))
+    streams = []
+    for i, (nchan, bw, lo, nf, ants) in enumerate(subbands):
+        cfg.subbands.append(cfgmod.SubbandEntry(nchan, bw, lo, nf, ants))
+        F = 2 * nchan
+        s = synth.make_data(nant, (nsub * nf + 1) * F, 40 + i)
+        for j in range(nant):
+            s[j].tofile(str(tmp_path / f"a{j}_sb{i}.bin"))
+        streams.append(s)
+    (tmp_path / "test.conf").write_text(cfgmod.format_config(cfg))
+    outs = {}
+    for mode in ([], ["--launch"]):
+        r = subprocess.run([FXBATCH, "./test.conf", "--binary", *mode], cwd=tmp_path, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs[tuple(mode)] = [visio.read_binary(str(tmp_path / f"vis_sb{i}.bin")) for i in range(len(subbands))]
+    for a, b in zip(outs[()], outs[("--launch",)]):
+        assert np.array_equal(a["cross"], b["cross"]) and np.array_equal(a["cross_counts"], b["cross_counts"])
+    for i, (nchan, bw, lo, nf, ants) in enumerate(subbands):
+        idx = list(range(nant)) if ants is None else ants
+        F = 2 * nchan
+        vis = cnt = 0
+        for s in range(nsub):
+            p = orc.plan(nchan, lo, bw, origin=s * nf)
+            buf = np.ascontiguousarray(streams[i][idx, s * nf * F // 2: (s * nf + nf + 1) * F // 2])
+            r = orc.process_f32(p, buf, delays[idx], fileoff[idx], (s * nf + np.arange(nf)).astype(np.int32))
+            vis = vis + r["vis"].astype(np.complex128)
+            cnt = cnt + r["count"]
+        d = outs[()][i]
+        assert d["nant"] == len(idx) and d["nchan"] == nchan and d["flags"] == 1
+        assert np.array_equal(d["cross_counts"], cnt)
+        live = cnt > 0
+        want = vis[live] / cnt[live, None, None]
+        err = np.linalg.norm(d["cross"][live] - want, axis=-1) / np.linalg.norm(want, axis=-1)
+        assert err.max() <= 1e-5
+    # the last run was the --launch one
+    log = (tmp_path / "runtime.fxbatch.log").read_text()
+    assert "plain launches" in log and log.count("subint ") == nsub
