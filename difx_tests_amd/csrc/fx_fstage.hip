@@ -293,20 +293,26 @@ __device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __rest
 // Plan V = 0 and, where a second one exists, V = 1; the autotuner picks per
 // context (fx_autotune).
 template <int LG, int V = 0> struct FPlan;
-template <> struct FPlan<9, 0>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1; };
-template <> struct FPlan<10, 0> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1; };
-template <> struct FPlan<10, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 16, R3 = 1; };
-template <> struct FPlan<11, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1; };
-template <> struct FPlan<11, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 16, R2 = 16, R3 = 1; };
-template <> struct FPlan<12, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
-template <> struct FPlan<12, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 8; };
-template <> struct FPlan<13, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2; };
-template <> struct FPlan<13, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 16, R2 = 16, R3 = 4; };
-template <> struct FPlan<14, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4; };
-template <> struct FPlan<14, 1> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 8,  R3 = 8; };
+template <> struct FPlan<9, 0>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1, KIND = 0; };
+template <> struct FPlan<10, 0> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1, KIND = 0; };
+template <> struct FPlan<10, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 16, R3 = 1, KIND = 0; };
+template <> struct FPlan<11, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1, KIND = 0; };
+template <> struct FPlan<11, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 16, R2 = 16, R3 = 1, KIND = 0; };
+template <> struct FPlan<12, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, KIND = 0; };
+template <> struct FPlan<12, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 8, KIND = 0; };
+template <> struct FPlan<13, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, KIND = 0; };
+template <> struct FPlan<13, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 16, R2 = 16, R3 = 4, KIND = 0; };
+template <> struct FPlan<14, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, KIND = 0; };
+template <> struct FPlan<14, 1> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 8,  R3 = 8, KIND = 0; };
+// KIND 0: Stockham passes; KIND 1: decimation in frequency with in-place
+// passes (dif_pass); KIND 2: Stockham with the last radix-2 pass done across
+// the two half-waves (v_permlane32_swap) instead of through LDS.
+template <> struct FPlan<12, 2> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, KIND = 1; };
+template <> struct FPlan<13, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, KIND = 2; };
+template <> struct FPlan<14, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, KIND = 1; };
 
 // number of plans per FFT length
-__host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : 2; }
+__host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : (lg >= 12 && lg <= 14) ? 3 : 2; }
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
@@ -315,6 +321,7 @@ __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 template <int LG, int V = 0, bool CPLX = false> struct Geo {
     using PL_ = FPlan<LG, V>;
     static constexpr int LGV = LG;
+    static constexpr int KIND = PL_::KIND;
     static constexpr int F = 1 << LG;
     static constexpr int NCH = CPLX ? F : F / 2;
     static constexpr int STRIDE = 1 << ((CPLX ? LG : LG - 1) / 2);
@@ -457,6 +464,65 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
 }
 
 // --------------------------------------------------------------------------
+// Decimation-in-frequency pass, in place: blocks of L points, butterflies on
+// the R points j + r*S (S = L/R) of a block, then the twiddle W_L^(j*k) on
+// output k, written back to the points the thread read.  No thread touches
+// another's points within a pass, so a pass needs one barrier (after it)
+// instead of the two of a Stockham pass.  After the passes the point at
+// position sum_p k_p*S_p holds frequency sum_p k_p*(R_0*...*R_(p-1)).
+
+template <class Gm, int R, int L>
+__device__ __forceinline__ void dif_pass(float2* buf, const float2* __restrict__ tw, int tid)
+{
+    constexpr int F = Gm::F, NT = Gm::NT;
+    constexpr int S = L / R;
+    constexpr int T = F / R, PER = T / NT;
+    static_assert(PER * NT == T, "plan/thread mismatch");
+    static_assert(S > 1, "the last pass has its own code");
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const int q = tid + m * NT;
+        const int j = q & (S - 1);
+        const int base = (q - j) * R + j;         // (q / S) * L + j
+        float2 x[R];
+        if constexpr (S % 16 == 0) {
+            // S a multiple of 16: pad(base + r*S) = pad(base) + r*(S + S/16)
+            float2* p = buf + pad(base);
+#pragma unroll
+            for (int r = 0; r < R; r++) x[r] = p[r * (S + S / 16)];
+            dft<R>(x);
+            twiddle_pow<R>(x, tw, j * (F / L));
+#pragma unroll
+            for (int r = 0; r < R; r++) p[r * (S + S / 16)] = x[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) x[r] = buf[pad(base + r * S)];
+            dft<R>(x);
+            twiddle_pow<R>(x, tw, j * (F / L));
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[pad(base + r * S)] = x[r];
+        }
+    }
+    __syncthreads();
+}
+
+// Frequency of the last DIF pass's butterfly q (q < F/R_last), output k = 0:
+// the digits of q in radices (R0, R1[, R2]), most significant first, reversed.
+template <class Gm>
+__device__ __forceinline__ int dif_freq(int q)
+{
+    constexpr int R0 = Gm::R0, R1 = Gm::R1;
+    if constexpr (Gm::NP == 4) {
+        constexpr int R2 = Gm::R2;
+        const int k2 = q % R2, k1 = (q / R2) % R1, k0 = q / (R1 * R2);
+        return k0 + R0 * (k1 + R1 * k2);
+    } else {
+        const int k1 = q % R1, k0 = q / R1;
+        return k0 + R0 * k1;
+    }
+}
+
+// --------------------------------------------------------------------------
 // The fused F-stage kernel: one block = one antenna, one polarisation, G
 // consecutive FFTs.
 
@@ -465,6 +531,8 @@ __global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_e
 void fx_fstage_kernel(FStageArgs args)
 {
     using Gm = Geo<LG, V, CPLX>;
+    constexpr bool DIF = (Gm::KIND == 1);
+    constexpr bool XL = (Gm::KIND == 2);
     constexpr int F = Gm::F, NCH = Gm::NCH, NT = Gm::NT, FP = Gm::FP;
     constexpr int STRIDE = Gm::STRIDE, SUB = Gm::SUB, NSTEP = Gm::NSTEP, NFSTEP = Gm::NFSTEP;
     constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2, RL = Gm::RL, PL = Gm::PL;
@@ -556,6 +624,20 @@ void fx_fstage_kernel(FStageArgs args)
     Prefetch cur;
     fetch(0, cur);
 
+    // XL: lanes 0-31 of wave w own the third-pass butterflies k = 32w + lane,
+    // lanes 32-63 the butterflies 256 + k; their outputs pair up in the last
+    // radix-2 step X[c] = A[c] + W^c B[c], c = k + 256*rho.  W^(k + 256*rho) =
+    // W^k (-j)^up W_32^r with rho = r + 8*up, so W^k (-j)^up is fixed per lane.
+    [[maybe_unused]] float2 xl_wk = make_float2(1.0f, 0.0f);
+    if constexpr (XL) {
+        static_assert(LG == 13 && NT == 512 && R0 == 16 && R1 == 16 && R2 == 16 && RL == 2 && STRIDE == 64,
+                      "cross-lane last pass laid out for 16*16*16*2");
+        const int up = (threadIdx.x >> 5) & 1;
+        const int k = (threadIdx.x >> 6) * 32 + (threadIdx.x & 31);
+        const float2 w = args.tw[k];
+        xl_wk = up ? make_float2(w.y, -w.x) : w;
+    }
+
 #pragma unroll 1
     for (int gi = 0; gi < G; gi++) {
         // Redefine tid for the compiler on every FFT: the LDS addresses derived
@@ -615,9 +697,16 @@ void fx_fstage_kernel(FStageArgs args)
                     x[r] = cmul(v, cmul(rs, sp[r * (T0 / SUB)]));
                 }
                 dft<R0>(x);
-                float2* wr = buf + pad(i * R0);
+                if constexpr (DIF) {
+                    twiddle_pow<R0>(x, args.tw, i);
+                    float2* wr = buf + pad(i);
 #pragma unroll
-                for (int r = 0; r < R0; r++) wr[r] = x[r];
+                    for (int r = 0; r < R0; r++) wr[r * (T0 + T0 / 16)] = x[r];
+                } else {
+                    float2* wr = buf + pad(i * R0);
+#pragma unroll
+                    for (int r = 0; r < R0; r++) wr[r] = x[r];
+                }
             }
         } else {
             const uint8_t* bl = reinterpret_cast<const uint8_t*>(bytes);
@@ -641,32 +730,114 @@ void fx_fstage_kernel(FStageArgs args)
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (T0 / SUB)]);
                 }
                 dft<R0>(x);
-                float2* wr = buf + pad(i * R0);
+                if constexpr (DIF) {
+                    twiddle_pow<R0>(x, args.tw, i);
+                    float2* wr = buf + pad(i);
 #pragma unroll
-                for (int r = 0; r < R0; r++) wr[r] = x[r];
+                    for (int r = 0; r < R0; r++) wr[r * (T0 + T0 / 16)] = x[r];
+                } else {
+                    float2* wr = buf + pad(i * R0);
+#pragma unroll
+                    for (int r = 0; r < R0; r++) wr[r] = x[r];
+                }
             }
         }
         __syncthreads();
 
+        if constexpr (XL) {
+            mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
+            // third pass (radix 16 on sub-transforms of 256), butterflies k
+            // and 256 + k in the two half-waves; its outputs stay in registers
+            const int up = (tid >> 5) & 1;
+            const int k = (tid >> 6) * 32 + (tid & 31);
+            constexpr int T2 = F / R2;
+            float2 x[16];
+            const float2* rd = buf + pad(k + 256 * up);
+#pragma unroll
+            for (int r = 0; r < 16; r++) x[r] = rd[r * (T2 + T2 / 16)];
+            twiddle_pow<16>(x, args.tw, k * (F / (256 * 16)));
+            dft<16>(x);
+            // x[r] is A[k + 256 r] (lanes 0-31) or B[k + 256 r] (lanes 32-63):
+            // trade halves so each lane holds A and B of rho = r + 8*up
+            float2 av[8], bv[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const auto sx = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[r].x), __float_as_uint(x[r + 8].x),
+                                                                 false, false);
+                const auto sy = __builtin_amdgcn_permlane32_swap(__float_as_uint(x[r].y), __float_as_uint(x[r + 8].y),
+                                                                 false, false);
+                av[r] = make_float2(__uint_as_float(sx[0]), __uint_as_float(sy[0]));
+                bv[r] = make_float2(__uint_as_float(sx[1]), __uint_as_float(sy[1]));
+            }
+            // last radix-2 step and fractional-delay rotation; c = k + 256*rho:
+            // c % 64 = k % 64, c / 64 = k / 64 + 32 up + 4 r
+            const float2 frs = frsub[k & (STRIDE - 1)];
+            const float2* frb = frstep + (k >> 6) + 32 * up;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const float2 wb = cmul(bv[r], twc(xl_wk, r));
+                const int c = k + 256 * (r + 8 * up);
+                {
+                    const float2 y = cmul(cmul(frs, frb[4 * r]), cadd(av[r], wb));
+                    float si = 1.0f;
+                    if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
+                    outr.put_last(r * RO, y, si);
+                    if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
+                }
+                if constexpr (CPLX) {                   // channel c + F/2 = a - W b
+                    const float2 y = cmul(cmul(frs, frb[4 * r + 64]), csub(av[r], wb));
+                    float si = 1.0f;
+                    if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c + F / 2];
+                    outr.put_last(r * RO + 1, y, si);
+                    if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c + F / 2] = y;
+                }
+            }
+        } else {
         // ---- middle passes
-        mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
-        if constexpr (Gm::NP == 4) mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
+        if constexpr (DIF) {
+            dif_pass<Gm, R1, F / R0>(buf, args.tw, tid);
+            if constexpr (Gm::NP == 4) dif_pass<Gm, R2, F / (R0 * R1)>(buf, args.tw, tid);
+        } else {
+            mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
+            if constexpr (Gm::NP == 4) mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
+        }
 
         // ---- last pass: radix RL, the channels (real input: bins < nchan),
         // frac-sample rotation
+        // Stockham: butterfly i = tid + m*NT reads pad(i) + r*(TL + TL/16)
+        // = pad(tid) + m*(NT + NT/16) + ... and its channels c = i + r*PL have
+        // c % STRIDE = tid % STRIDE and c / STRIDE = tid / STRIDE + (m*NT +
+        // r*PL) / STRIDE, so every address is a per-thread base plus a constant
+        static_assert(NT % 16 == 0 && NT % STRIDE == 0 && PL % STRIDE == 0, "constant offsets");
+        const float2* rdb = buf + pad(tid);
+        const float2 frs = frsub[tid & (STRIDE - 1)];
+        const float2* frb = frstep + tid / STRIDE;
 #pragma unroll
         for (int m = 0; m < PERL; m++) {
             const int i = tid + m * NT;
             float2 x[RL];
-            const float2* rd = buf + pad(i);
+            int c0;
+            if constexpr (DIF) {
+                // the RL consecutive points of butterfly i; no twiddle after the last DIF pass
+                static_assert(16 % RL == 0, "pad(i*RL + r) = pad(i*RL) + r");
+                const float2* rd = buf + pad(i * RL);
 #pragma unroll
-            for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
-            twiddle_pow<RL>(x, args.tw, i);
+                for (int r = 0; r < RL; r++) x[r] = rd[r];
+                c0 = dif_freq<Gm>(i);
+            } else {
+                const float2* rd = rdb + m * (NT + NT / 16);
+#pragma unroll
+                for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
+                twiddle_pow<RL>(x, args.tw, i);
+                c0 = i;
+            }
             dft<RL, !CPLX>(x);
 #pragma unroll
             for (int r = 0; r < RO; r++) {
-                const int c = i + r * PL;
-                const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
+                const int c = c0 + r * PL;
+                float2 fr;
+                if constexpr (DIF) fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
+                else fr = cmul(frs, frb[(m * NT + r * PL) / STRIDE]);
                 const float2 y = cmul(fr, x[r]);
                 float si = 1.0f;
                 if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
@@ -674,6 +845,7 @@ void fx_fstage_kernel(FStageArgs args)
                 if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
             }
         }
+        }   // not XL
     }
 
     // ---- stores: corner-turned [chunk][chan][group in chunk][Spad] rows of G FFTs
@@ -681,10 +853,14 @@ void fx_fstage_kernel(FStageArgs args)
 #pragma unroll
     for (int m = 0; m < PERL; m++) {
         const int i = tid + m * NT;
+        int c0 = i;
+        if constexpr (DIF) c0 = dif_freq<Gm>(i);
 #pragma unroll
         for (int r = 0; r < RO; r++) {
-            const int c = i + r * PL;
+            int c = c0 + r * PL;
             const int ou = m * RO + r;
+            // XL: output m*RO + r is channel k + 256 (m + 8 up) (+ F/2 for r = 1)
+            if constexpr (XL) c = ((tid >> 6) * 32 + (tid & 31)) + 256 * (m + 8 * ((tid >> 5) & 1)) + r * (F / 2);
             const long long row = xrow0 + (long long)c * args.kc * p.Spad;
             if constexpr (PREC == FX_PREC_F16) {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
@@ -1058,16 +1234,21 @@ template <int LG>
 static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
 {
     if (a.p.iscomplex) {
+        if constexpr (nplans(LG) > 2)
+            if (a.plan == 2) return launch_lgv<LG, 2, true>(a, prec, s);
         if constexpr (nplans(LG) > 1)
             if (a.plan == 1) return launch_lgv<LG, 1, true>(a, prec, s);
         return launch_lgv<LG, 0, true>(a, prec, s);
     }
+    if constexpr (nplans(LG) > 2)
+        if (a.plan == 2) return launch_lgv<LG, 2, false>(a, prec, s);
     if constexpr (nplans(LG) > 1)
         if (a.plan == 1) return launch_lgv<LG, 1, false>(a, prec, s);
     return launch_lgv<LG, 0, false>(a, prec, s);
 }
 
-int fstage_plans(int lg) { return lg == kEoLG ? nplans(kEoLG - 1) : nplans(lg); }
+// the 32768-point halves use the Stockham plans only
+int fstage_plans(int lg) { return lg == kEoLG ? 2 : nplans(lg); }
 
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
 {
