@@ -438,8 +438,11 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
 #pragma unroll
         for (int r = 0; r < R; r++) x[m][r] = rd[r * (T + T / 16)];
         if (TW_LDS) {
+            // tw1 is k-major with row stride R + 1: a per-lane base and an
+            // immediate offset per r, and 16 lanes hit 16 distinct banks
+            const float2* tk = tw1 + k * (R + 1);
 #pragma unroll
-            for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tw1[r * k]);
+            for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tk[r]);
         } else {
             twiddle_pow<R>(x[m], tw, k * (TWS * F / (P * R)));
         }
@@ -548,10 +551,12 @@ void fx_fstage_kernel(FStageArgs args)
     __shared__ __align__(16) float2 buf[FP];
     // real input: [time slot][byte] -> level of this pol, and the packed bytes
     // of the FFT window; complex input reads its int8 pairs from global memory
-    __shared__ __align__(16) float lut[CPLX ? 1 : 2][256];
+    // level of this pol at [byte][time slot]: the two lanes that read one byte
+    // (its two time slots) hit adjacent words, not one bank
+    __shared__ __align__(16) float lut[CPLX ? 2 : 512];
     __shared__ __align__(16) uint32_t bytes[CPLX ? 4 : F / 8 + 4];
     __shared__ __align__(16) float2 tab[Gm::NTAB];       // rotator tables of this (FFT, antenna)
-    __shared__ float2 tw1[NTW1];                         // exp(-2 pi i m / (R0 R1))
+    __shared__ float2 tw1[R0 * (R1 + 1)];               // [k][r] = exp(-2 pi i k r / (R0 R1)), rows R1 + 1 apart
     const float2* rsub = tab;
     const float2* rstep = tab + SUB;
     const float2* frsub = tab + SUB + NSTEP;
@@ -578,14 +583,15 @@ void fx_fstage_kernel(FStageArgs args)
     const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
     int tid = threadIdx.x;
 
-    for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT)
+        tw1[e] = args.tw[((e / (R1 + 1)) * (e % (R1 + 1))) % NTW1 * (F / NTW1)];
     if constexpr (!CPLX) {
         for (int e = tid; e < 256; e += NT) {
             // pol p of time slot t sits at bits 2*(2t+p) (kernel.c:195-200)
             const float hi = 3.3359f;
             const float lv[4] = {-hi, -1.0f, 1.0f, hi};
-            lut[0][e] = lv[(e >> (2 * pol)) & 3];
-            lut[1][e] = lv[(e >> (4 + 2 * pol)) & 3];
+            lut[2 * e] = lv[(e >> (2 * pol)) & 3];
+            lut[2 * e + 1] = lv[(e >> (4 + 2 * pol)) & 3];
         }
     }
 
@@ -718,7 +724,7 @@ void fx_fstage_kernel(FStageArgs args)
                 // time slot is fixed per thread and the byte, step-rotator and
                 // LDS write addresses are a base plus r times a constant
                 const int q0 = o + i;
-                const float* lt = lut[q0 & 1];
+                const float* lt = lut + (q0 & 1);
                 const uint8_t* bp = bl + ((q0 >> 1) - bbase);
                 const float2* sp = rstep + i / SUB;
                 float2 x[R0];
@@ -726,7 +732,7 @@ void fx_fstage_kernel(FStageArgs args)
                 for (int r = 0; r < R0; r++) {
                     // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
                     // fine rotator first, then one contracted complex product
-                    const float l = lt[bp[r * (T0 / 2)]];
+                    const float l = lt[2 * bp[r * (T0 / 2)]];
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (T0 / SUB)]);
                 }
                 dft<R0>(x);
@@ -931,9 +937,9 @@ void fx_fstage_eo_kernel(FStageArgs args)
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    __shared__ __align__(16) float lut[2][256];
+    __shared__ __align__(16) float lut[512];             // [byte][time slot], as in fx_fstage_kernel
     __shared__ __align__(16) float2 tab[NTAB];
-    __shared__ float2 tw1[NTW1];
+    __shared__ float2 tw1[R0 * (R1 + 1)];
     const float2* rsub = tab;
     const float2* rstep = tab + SUB;
     const float2* frsub = tab + SUB + NSTEP;
@@ -953,12 +959,13 @@ void fx_fstage_eo_kernel(FStageArgs args)
     const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
     int tid = threadIdx.x;
 
-    for (int e = tid; e < NTW1; e += NT) tw1[e] = args.tw[e * (F / NTW1)];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT)
+        tw1[e] = args.tw[((e / (R1 + 1)) * (e % (R1 + 1))) % NTW1 * (F / NTW1)];
     for (int e = tid; e < 256; e += NT) {
         const float hi = 3.3359f;
         const float lv[4] = {-hi, -1.0f, 1.0f, hi};
-        lut[0][e] = lv[(e >> (2 * pol)) & 3];
-        lut[1][e] = lv[(e >> (4 + 2 * pol)) & 3];
+        lut[2 * e] = lv[(e >> (2 * pol)) & 3];
+        lut[2 * e + 1] = lv[(e >> (4 + 2 * pol)) & 3];
     }
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
 
@@ -999,12 +1006,12 @@ void fx_fstage_eo_kernel(FStageArgs args)
                 const int n0 = 2 * i + h;
                 const float2 rs = rsub[n0 & (SUB - 1)];
                 const float2* sp = rstep + n0 / SUB;
-                const float* lt = lut[(o + h) & 1];
+                const float* lt = lut + ((o + h) & 1);
                 const uint8_t* bp = src + ((o + h) >> 1) + i;
                 float2 x[R0];
 #pragma unroll
                 for (int r = 0; r < R0; r++) {
-                    const float l = lt[bp[r * T0]];
+                    const float l = lt[2 * bp[r * T0]];
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (2 * T0 / SUB)]);
                 }
                 dft<R0>(x);
@@ -1085,11 +1092,12 @@ __global__ __launch_bounds__(Geo<kEoLG - 1>::NT) void fx_fft_c2c_eo_kernel(const
     constexpr int R0 = Gh::R0, R1 = Gh::R1, R2 = Gh::R2, RL = Gh::RL;
     constexpr int T0 = H / R0, TL = H / RL, PERL = TL / NT, PL = H / RL;
     __shared__ __align__(16) float2 buf[FP];
-    __shared__ float2 tw1[R0 * R1];
+    __shared__ float2 tw1[R0 * (R1 + 1)];
     const float2* x_in = in + (long long)blockIdx.x * F;
     float2* x_out = out + (long long)blockIdx.x * F;
     int tid = threadIdx.x;
-    for (int e = tid; e < R0 * R1; e += NT) tw1[e] = tw[e * (F / (R0 * R1))];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT)
+        tw1[e] = tw[((e / (R1 + 1)) * (e % (R1 + 1))) % (R0 * R1) * (F / (R0 * R1))];
     float2 ev[PERL][RL];
 #pragma unroll 1
     for (int h = 0; h < 2; h++) {
@@ -1147,11 +1155,12 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* _
     constexpr int R0 = Gm::R0, R1 = Gm::R1, R2 = Gm::R2, RL = Gm::RL, PL = Gm::PL;
     constexpr int T0 = F / R0, PER0 = T0 / NT, TL = F / RL, PERL = TL / NT;
     __shared__ __align__(16) float2 buf[FP];
-    __shared__ float2 tw1[R0 * R1];
+    __shared__ float2 tw1[R0 * (R1 + 1)];
     const float2* x_in = in + (long long)blockIdx.x * F;
     float2* x_out = out + (long long)blockIdx.x * F;
     const int tid = threadIdx.x;
-    for (int e = tid; e < R0 * R1; e += NT) tw1[e] = tw[e * (F / (R0 * R1))];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT)
+        tw1[e] = tw[((e / (R1 + 1)) * (e % (R1 + 1))) % (R0 * R1) * (F / (R0 * R1))];
 #pragma unroll
     for (int m = 0; m < PER0; m++) {
         const int i = tid + m * NT;
