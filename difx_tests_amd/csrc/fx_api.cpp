@@ -1105,33 +1105,40 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
     rc = iter_upload(c, it.data(), f, c->stream);
     if (rc) return rc;
 
-    // F-stage radix plans
-    double best_f = 1e30;
-    int best_plan = 0;
-    for (int v = 0; rc == FX_OK && v < fstage_plans(c->p.lg); v++) {
-        c->fplan = v;
-        double ms = 0.0;
-        rc = tune_time(c, reps, &ms, [&] { return fstage_enqueue(c, f, c->stream); });
-        if (rc == FX_OK && ms < best_f) {
-            best_f = ms;
-            best_plan = v;
+    // F-stage radix plans and X-engine tiles: candidates timed round-robin,
+    // the best of the rounds kept (differences of a few percent must not
+    // drown in the spread of single timings)
+    constexpr int kRounds = 3;
+    const int nplan = fstage_plans(c->p.lg);
+    std::vector<double> fms(nplan, 1e30);
+    for (int round = 0; rc == FX_OK && round < kRounds; round++)
+        for (int v = 0; rc == FX_OK && v < nplan; v++) {
+            c->fplan = v;
+            double ms = 0.0;
+            rc = tune_time(c, reps, &ms, [&] { return fstage_enqueue(c, f, c->stream); });
+            if (rc == FX_OK) fms[v] = std::min(fms[v], ms);
         }
-    }
+    int best_plan = 0;
+    for (int v = 1; v < nplan; v++)
+        if (fms[v] < fms[best_plan]) best_plan = v;
+    double best_f = fms[best_plan];
     c->fplan = best_plan;
     // X-engine tiles over one subint
-    double best_x = 1e30;
-    int best_tile = 1;
     const int tiles[3] = {1, 2, 4};
-    for (int t = 0; rc == FX_OK && t < 3; t++) {
-        if (tiles[t] > 1 && c->p.nblk < tiles[t] - 1) continue;       // tile far larger than the signal count
-        c->xsb = tiles[t];
-        double ms = 0.0;
-        rc = tune_time(c, reps, &ms, [&] { return xstage_enqueue(c, 0, f.npad, c->stream); });
-        if (rc == FX_OK && ms < best_x) {
-            best_x = ms;
-            best_tile = tiles[t];
+    double xms[3] = {1e30, 1e30, 1e30};
+    for (int round = 0; rc == FX_OK && round < kRounds; round++)
+        for (int t = 0; rc == FX_OK && t < 3; t++) {
+            if (tiles[t] > 1 && c->p.nblk < tiles[t] - 1) continue;   // tile far larger than the signal count
+            c->xsb = tiles[t];
+            double ms = 0.0;
+            rc = tune_time(c, reps, &ms, [&] { return xstage_enqueue(c, 0, f.npad, c->stream); });
+            if (rc == FX_OK) xms[t] = std::min(xms[t], ms);
         }
-    }
+    int bt = 0;
+    for (int t = 1; t < 3; t++)
+        if (xms[t] < xms[bt]) bt = t;
+    double best_x = xms[bt];
+    const int best_tile = tiles[bt];
     c->xsb = best_tile;
     // subints per X-engine launch: whole steps of k F-stages and one X-engine
     // launch over their slots; the smallest k within 3% of the best time per

@@ -184,7 +184,8 @@ int fx_get_option(const fx_context *ctx, int option, int *value);
 /* Autotuner: times every F-stage radix plan, every X-engine tile and 1, 2,
  * 4, ... subints per X-engine launch (up to the FFT capacity) on this
  * context's uploaded input and delay model, numffts FFTs per subint, each
- * candidate once untimed then `reps` times; keeps the fastest of each (for the
+ * candidate once untimed then `reps` times (plans and tiles in three
+ * round-robin rounds, best round kept); keeps the fastest of each (for the
  * launch count, timed as whole steps of that many F-stages and one X-engine
  * launch, the smallest within 3% of the best time per subint) and zeroes the
  * visibilities and counts.  report (may be NULL) receives {plan, tile,
