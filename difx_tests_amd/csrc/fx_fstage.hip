@@ -347,6 +347,19 @@ template <int LG, int V = 0, bool CPLX = false> struct Geo {
 
 __device__ __forceinline__ int pad(int n) { return n + (n >> 4); }  // LDS bank skew
 
+// Load of data that no kernel writes while this one runs, through the
+// constant address space: at a wave-uniform address it is a scalar load.
+__device__ __forceinline__ float2 load_const(const float2* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const float cfloat;
+    const cfloat* q = (const cfloat*)p;
+    return make_float2(q[0], q[1]);
+#else
+    return *p;
+#endif
+}
+
 // Per-thread storage of the G spectra of the channels a thread owns.
 template <int PREC, int G, int NOUT> struct OutRegs;
 // The G FFTs of a group pass through a register shift chain: after each FFT
@@ -716,6 +729,11 @@ void fx_fstage_kernel(FStageArgs args)
             }
         } else {
             const uint8_t* bl = reinterpret_cast<const uint8_t*>(bytes);
+            // SUB a multiple of the wave size: the step rotator index i / SUB is
+            // the same in every lane of a wave, so it is read with scalar loads
+            // from the rotator table in L2 instead of the LDS
+            constexpr bool SSTEP = (SUB % 64 == 0) && (NT % SUB == 0);
+            const float2* rstep_g = args.rtab + ((long long)fft * p.nant + ant) * Gm::NTAB + SUB;
 #pragma unroll
             for (int m = 0; m < PER0; m++) {
                 const int i = tid + m * NT;
@@ -727,13 +745,15 @@ void fx_fstage_kernel(FStageArgs args)
                 const float* lt = lut + (q0 & 1);
                 const uint8_t* bp = bl + ((q0 >> 1) - bbase);
                 const float2* sp = rstep + i / SUB;
+                const float2* spg = rstep_g + __builtin_amdgcn_readfirstlane(i / SUB);
                 float2 x[R0];
 #pragma unroll
                 for (int r = 0; r < R0; r++) {
                     // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
                     // fine rotator first, then one contracted complex product
                     const float l = lt[2 * bp[r * (T0 / 2)]];
-                    x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (T0 / SUB)]);
+                    const float2 st = SSTEP ? load_const(spg + r * (T0 / SUB)) : sp[r * (T0 / SUB)];
+                    x[r] = cmul(make_float2(l * rs.x, l * rs.y), st);
                 }
                 dft<R0>(x);
                 if constexpr (DIF) {
