@@ -347,6 +347,15 @@ template <int LG, int V = 0, bool CPLX = false> struct Geo {
 
 __device__ __forceinline__ int pad(int n) { return n + (n >> 4); }  // LDS bank skew
 
+// Each byte of w holds time slots t = 0, 1 of both pols, pol p of slot t at
+// bits 2*(2t+p) (kernel.c:195-200).  Returns, per byte, the two codes of pol
+// p packed as slot0 | slot1 << 2.
+__device__ __forceinline__ uint32_t pol_codes(uint32_t w, int pol)
+{
+    const uint32_t t = w >> (2 * pol);
+    return (t & 0x03030303u) | ((t >> 2) & 0x0c0c0c0cu);
+}
+
 // Load of data that no kernel writes while this one runs, through the
 // constant address space: at a wave-uniform address it is a scalar load.
 __device__ __forceinline__ float2 load_const(const float2* p)
@@ -562,11 +571,12 @@ void fx_fstage_kernel(FStageArgs args)
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    // real input: [time slot][byte] -> level of this pol, and the packed bytes
-    // of the FFT window; complex input reads its int8 pairs from global memory
-    // level of this pol at [byte][time slot]: the two lanes that read one byte
-    // (its two time slots) hit adjacent words, not one bank
-    __shared__ __align__(16) float lut[CPLX ? 2 : 512];
+    // real input: the FFT window with each byte recoded to this pol's two
+    // codes (pol_codes), and the level table lut[2*code pair + time slot]: its
+    // 32 words sit in 32 banks, so lanes that read one entry share a broadcast
+    // and lanes that read different entries never conflict.  Complex input
+    // reads its int8 pairs from global memory.
+    __shared__ __align__(16) float lut[CPLX ? 2 : 32];
     __shared__ __align__(16) uint32_t bytes[CPLX ? 4 : F / 8 + 4];
     __shared__ __align__(16) float2 tab[Gm::NTAB];       // rotator tables of this (FFT, antenna)
     __shared__ float2 tw1[R0 * (R1 + 1)];               // [k][r] = exp(-2 pi i k r / (R0 R1)), rows R1 + 1 apart
@@ -599,12 +609,11 @@ void fx_fstage_kernel(FStageArgs args)
     for (int e = tid; e < R0 * (R1 + 1); e += NT)
         tw1[e] = args.tw[((e / (R1 + 1)) * (e % (R1 + 1))) % NTW1 * (F / NTW1)];
     if constexpr (!CPLX) {
-        for (int e = tid; e < 256; e += NT) {
-            // pol p of time slot t sits at bits 2*(2t+p) (kernel.c:195-200)
+        if (tid < 32) {
+            // code pair e = slot-0 code | slot-1 code << 2; levels of kernel.c:191-192
             const float hi = 3.3359f;
             const float lv[4] = {-hi, -1.0f, 1.0f, hi};
-            lut[2 * e] = lv[(e >> (2 * pol)) & 3];
-            lut[2 * e + 1] = lv[(e >> (4 + 2 * pol)) & 3];
+            lut[tid] = lv[((tid >> 1) >> (2 * (tid & 1))) & 3];
         }
     }
 
@@ -692,7 +701,7 @@ void fx_fstage_kernel(FStageArgs args)
 #pragma unroll
             for (int w = 0; w < WPT; w++) {
                 const int k = tid + w * NT;
-                if (k < F / 8 + 4) bytes[k] = cur.w[w];
+                if (k < F / 8 + 4) bytes[k] = pol_codes(cur.w[w], pol);
             }
         }
         __syncthreads();

@@ -194,6 +194,32 @@ def test_vis_fp16_vs_fp64_oracle(nchan, nant, nf):
     assert _rel(autos, ref["autos"]).max() <= 1e-3
 
 
+@pytest.mark.parametrize("xtile", [1, 2, 4])
+@pytest.mark.parametrize("prec,nf", [("f32", 23), ("f32", 40), ("f16", 40), ("f16", 75)])
+def test_vis_multi_step_k_pipeline(prec, nf, xtile):
+    # X-engine K loops of 3 to 10 chunks, odd and even (two chunks per
+    # pipeline step, the last one half empty when odd), with 80 signals in
+    # 3 blocks: panels that overhang the last block at tiles 2 and 4
+    nchan, nant = 256, 40
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 300 + nf + xtile, data="correlated")
+    with _engine(p, nant, nf, prec=prec, data=data) as e:
+        e.set_xtile(xtile)
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    if prec == "f32":
+        ref, tol = orc.process_f32(p, data, delays, fileoff, it), 1e-5
+    else:
+        ref, tol = orc.process_f64(p, data, delays, fileoff, it), 1e-3
+    assert np.array_equal(cc, ref["count"]) and np.array_equal(ac, ref["autocount"])
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= tol
+    alive = ac > 0
+    assert _rel(autos[alive], ref["autos"][alive]).max() <= tol
+
+
 @pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 64), (2048, 12, 32), (16384, 4, 24)])
 def test_vis_fp8_vs_fp64_oracle(nchan, nant, nf):
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 50 + nchan, data="correlated")
