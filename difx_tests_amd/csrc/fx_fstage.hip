@@ -293,23 +293,27 @@ __device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __rest
 // Plan V = 0 and, where a second one exists, V = 1; the autotuner picks per
 // context (fx_autotune).
 template <int LG, int V = 0> struct FPlan;
-template <> struct FPlan<9, 0>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1, KIND = 0; };
-template <> struct FPlan<10, 0> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1, KIND = 0; };
-template <> struct FPlan<10, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 16, R3 = 1, KIND = 0; };
-template <> struct FPlan<11, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1, KIND = 0; };
-template <> struct FPlan<11, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 16, R2 = 16, R3 = 1, KIND = 0; };
-template <> struct FPlan<12, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, KIND = 0; };
-template <> struct FPlan<12, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 8, KIND = 0; };
-template <> struct FPlan<13, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, KIND = 0; };
-template <> struct FPlan<13, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 16, R2 = 16, R3 = 4, KIND = 0; };
-template <> struct FPlan<14, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, KIND = 0; };
-template <> struct FPlan<14, 1> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 8,  R3 = 8, KIND = 0; };
+template <> struct FPlan<9, 0>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1, PADB = 4, KIND = 0; };
+template <> struct FPlan<10, 0> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1, PADB = 4, KIND = 0; };
+template <> struct FPlan<10, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 16, R3 = 1, PADB = 4, KIND = 0; };
+template <> struct FPlan<11, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 8,  R3 = 1, PADB = 4, KIND = 0; };
+template <> struct FPlan<11, 1> { static constexpr int NP = 3, R0 = 8,  R1 = 16, R2 = 16, R3 = 1, PADB = 4, KIND = 0; };
+template <> struct FPlan<12, 0> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, PADB = 5, KIND = 0; };
+template <> struct FPlan<12, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 8, PADB = 5, KIND = 0; };
+template <> struct FPlan<13, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, PADB = 5, KIND = 0; };
+template <> struct FPlan<13, 1> { static constexpr int NP = 4, R0 = 8,  R1 = 16, R2 = 16, R3 = 4, PADB = 4, KIND = 0; };
+template <> struct FPlan<14, 0> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, PADB = 5, KIND = 0; };
+template <> struct FPlan<14, 1> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 8,  R3 = 8, PADB = 5, KIND = 0; };
+// PADB: LDS skew of one point per 2^PADB points (pad), the one of 16 or 32
+// that measured fewer LDS bank-conflict cycles for the plan's access pattern
+// (the LDS has 64 banks of 4 bytes; ds_read2/write2 pairs and single 8-byte
+// accesses conflict differently, so no one skew suits every plan).
 // KIND 0: Stockham passes; KIND 1: decimation in frequency with in-place
 // passes (dif_pass); KIND 2: Stockham with the last radix-2 pass done across
 // the two half-waves (v_permlane32_swap) instead of through LDS.
-template <> struct FPlan<12, 2> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, KIND = 1; };
-template <> struct FPlan<13, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, KIND = 2; };
-template <> struct FPlan<14, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, KIND = 1; };
+template <> struct FPlan<12, 2> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, PADB = 4, KIND = 1; };
+template <> struct FPlan<13, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, PADB = 4, KIND = 2; };
+template <> struct FPlan<14, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, PADB = 4, KIND = 1; };
 
 // number of plans per FFT length
 __host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : (lg >= 12 && lg <= 14) ? 3 : 2; }
@@ -335,17 +339,25 @@ template <int LG, int V = 0, bool CPLX = false> struct Geo {
     static constexpr int RL = NP == 4 ? R3 : R2;            // last radix
     static constexpr int PL = F / RL;                       // transform length before the last pass
     static constexpr int NOUT = NCH / NT;                   // channels per thread per FFT
-    static constexpr int FP = F + F / 16;                   // padded LDS row
-    // waves per SIMD: at F >= 8192 the LDS allows two 8-wave blocks (or one
-    // 16-wave block) per CU
-    static constexpr int WPE = (LG >= 13 || (V == 1 && LG >= 12)) ? 4 : 3;
+    static constexpr int PADB = PL_::PADB;
+    static constexpr int PM = 1 << PADB;                    // points per skew step
+    static constexpr int FP = F + (F >> PADB);              // padded LDS row
+    // LDS index of point n.  pad(b + s) = pad(b) + pad(s) whenever
+    // (b mod PM) + (s mod PM) < PM; the butterfly addressing meets that with
+    // s = r*S for S a multiple of PM, or for S dividing PM and b mod PM < S,
+    // so every offset from a butterfly's base address is a constant.
+    static __host__ __device__ constexpr int pad(int n) { return n + (n >> PADB); }
+    // waves per SIMD: at F = 8192 the LDS allows two 8-wave blocks per CU, at
+    // F = 16384 one 16-wave block (at F = 4096 see fstage_wpe)
+    static constexpr int WPE = LG >= 13 ? 4 : 3;
     static_assert(R0 * R1 * R2 * R3 == F, "plan must factor F");
     static_assert((F / R0) % SUB == 0, "pass-0 rotator reuse needs F/R0 to be a multiple of sub");
     static_assert(R0 == 16 || (R0 == 8 && F / R0 >= 64), "pass-0 writes pad(i*R0 + r) = pad(i*R0) + r");
+    static_assert(NT % PM == 0, "last-pass reads pad(i + m*NT + r*TL) = pad(i) + pad(m*NT) + pad(r*TL)");
+    static_assert(PM % R0 == 0, "pass-0 writes pad(i*R0 + r) = pad(i*R0) + r");
     static_assert(NT <= 1024, "block too large");
 };
 
-__device__ __forceinline__ int pad(int n) { return n + (n >> 4); }  // LDS bank skew
 
 // Each byte of w holds time slots t = 0, 1 of both pols, pol p of slot t at
 // bits 2*(2t+p) (kernel.c:195-200).  Returns, per byte, the two codes of pol
@@ -448,17 +460,19 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
     constexpr int F = Gm::F, NT = Gm::NT, FP = Gm::FP;
     constexpr int T = F / R, PER = T / NT;
     static_assert(PER * NT == T, "plan/thread mismatch");
-    static_assert(T % 16 == 0, "padded addressing needs T to be a multiple of 16");
+    constexpr int PM = Gm::PM;
+    static_assert(T % PM == 0, "padded addressing needs T to be a multiple of PM");
+    static_assert(P % PM == 0 || (PM % P == 0 && (P * R) % PM == 0), "write offsets pad(r*P)");
     float2 x[PER][R];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
         const int i = tid + m * NT;
         const int k = i & (P - 1);
-        // T and P are multiples of 16, so pad(i + r*T) = pad(i) + r*(T + T/16):
-        // one base address and immediate offsets per butterfly
-        const float2* rd = buf + pad(i);
+        // T is a multiple of PM, so pad(i + r*T) = pad(i) + pad(r*T): one base
+        // address and immediate offsets per butterfly
+        const float2* rd = buf + Gm::pad(i);
 #pragma unroll
-        for (int r = 0; r < R; r++) x[m][r] = rd[r * (T + T / 16)];
+        for (int r = 0; r < R; r++) x[m][r] = rd[Gm::pad(r * T)];
         if (TW_LDS) {
             // tw1 is k-major with row stride R + 1: a per-lane base and an
             // immediate offset per r, and 16 lanes hit 16 distinct banks
@@ -476,14 +490,10 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
         const int i = tid + m * NT;
         const int k = i & (P - 1);
         const int j = (i - k) * R + k;
-        if constexpr (P % 16 == 0) {
-            float2* wr = buf + pad(j);
+        // j mod PM = k < P (or P a multiple of PM): pad(j + r*P) = pad(j) + pad(r*P)
+        float2* wr = buf + Gm::pad(j);
 #pragma unroll
-            for (int r = 0; r < R; r++) wr[r * (P + P / 16)] = x[m][r];
-        } else {
-#pragma unroll
-            for (int r = 0; r < R; r++) buf[pad(j + r * P)] = x[m][r];
-        }
+        for (int r = 0; r < R; r++) wr[Gm::pad(r * P)] = x[m][r];
     }
     __syncthreads();
 }
@@ -510,22 +520,22 @@ __device__ __forceinline__ void dif_pass(float2* buf, const float2* __restrict__
         const int j = q & (S - 1);
         const int base = (q - j) * R + j;         // (q / S) * L + j
         float2 x[R];
-        if constexpr (S % 16 == 0) {
-            // S a multiple of 16: pad(base + r*S) = pad(base) + r*(S + S/16)
-            float2* p = buf + pad(base);
+        if constexpr (S % Gm::PM == 0 || (Gm::PM % S == 0 && L % Gm::PM == 0)) {
+            // base mod PM = j < S (or S a multiple of PM): pad(base + r*S) = pad(base) + pad(r*S)
+            float2* p = buf + Gm::pad(base);
 #pragma unroll
-            for (int r = 0; r < R; r++) x[r] = p[r * (S + S / 16)];
+            for (int r = 0; r < R; r++) x[r] = p[Gm::pad(r * S)];
             dft<R>(x);
             twiddle_pow<R>(x, tw, j * (F / L));
 #pragma unroll
-            for (int r = 0; r < R; r++) p[r * (S + S / 16)] = x[r];
+            for (int r = 0; r < R; r++) p[Gm::pad(r * S)] = x[r];
         } else {
 #pragma unroll
-            for (int r = 0; r < R; r++) x[r] = buf[pad(base + r * S)];
+            for (int r = 0; r < R; r++) x[r] = buf[Gm::pad(base + r * S)];
             dft<R>(x);
             twiddle_pow<R>(x, tw, j * (F / L));
 #pragma unroll
-            for (int r = 0; r < R; r++) buf[pad(base + r * S)] = x[r];
+            for (int r = 0; r < R; r++) buf[Gm::pad(base + r * S)] = x[r];
         }
     }
     __syncthreads();
@@ -551,8 +561,18 @@ __device__ __forceinline__ int dif_freq(int q)
 // The fused F-stage kernel: one block = one antenna, one polarisation, G
 // consecutive FFTs.
 
+// Waves per SIMD the kernel is compiled for.  At F = 4096 the LDS admits four
+// 4-wave blocks per CU, which needs at most 128 VGPRs: real input with FP16
+// or FP32 output fits, FP8 output and complex input (twice the channels) do
+// not and keep three.
+template <int LG, int V, int PREC, bool CPLX>
+__host__ __device__ constexpr int fstage_wpe()
+{
+    return (LG == 12 && !CPLX && PREC != FX_PREC_F8) ? 4 : Geo<LG, V, CPLX>::WPE;
+}
+
 template <int LG, int V, int PREC, bool TAP, bool CPLX = false>
-__global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_eu(Geo<LG, V>::WPE)))
+__global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_eu(fstage_wpe<LG, V, PREC, CPLX>())))
 void fx_fstage_kernel(FStageArgs args)
 {
     using Gm = Geo<LG, V, CPLX>;
@@ -571,15 +591,32 @@ void fx_fstage_kernel(FStageArgs args)
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    // real input: the FFT window with each byte recoded to this pol's two
-    // codes (pol_codes), and the level table lut[2*code pair + time slot]: its
-    // 32 words sit in 32 banks, so lanes that read one entry share a broadcast
-    // and lanes that read different entries never conflict.  Complex input
-    // reads its int8 pairs from global memory.
-    __shared__ __align__(16) float lut[CPLX ? 2 : 32];
-    __shared__ __align__(16) uint32_t bytes[CPLX ? 4 : F / 8 + 4];
-    __shared__ __align__(16) float2 tab[Gm::NTAB];       // rotator tables of this (FFT, antenna)
-    __shared__ float2 tw1[R0 * (R1 + 1)];               // [k][r] = exp(-2 pi i k r / (R0 R1)), rows R1 + 1 apart
+    // The smaller tables (at F = 4096 in one LDS array: each __shared__ array
+    // is rounded up to 16 bytes, and the 16 bytes saved let four blocks share
+    // a CU; elsewhere as separate arrays, which measured faster at F = 8192):
+    //  * tw1[k][r] = exp(-2 pi i k r / (R0 R1)), rows R1 + 1 apart, the last
+    //    row's spare entry left out;
+    //  * tab: the rotator tables of this (FFT, antenna);
+    //  * bytes (real input): the FFT window with each byte recoded to this
+    //    pol's two codes (pol_codes); F samples span at most F/2 + 4 bytes
+    //    from the window's aligned start, so F/8 + 1 words;
+    //  * lut (real input): the level table lut[2*code pair + time slot]; its
+    //    32 words sit in 32 banks, so lanes that read one entry share a
+    //    broadcast and lanes that read different entries never conflict.
+    // Complex input reads its int8 pairs from global memory.
+    constexpr int NTW1L = R0 * (R1 + 1) - 1;
+    constexpr int NWIN = CPLX ? 4 : F / 8 + 1;
+    constexpr int NLUT = CPLX ? 2 : 32;
+    constexpr bool MERGE = LG == 12;
+    __shared__ __align__(16) float2 aux[MERGE ? NTW1L + Gm::NTAB + (NWIN + NLUT + 1) / 2 : 1];
+    __shared__ __align__(16) float2 tw1_s[MERGE ? 1 : NTW1L];
+    __shared__ __align__(16) float2 tab_s[MERGE ? 1 : Gm::NTAB];
+    __shared__ __align__(16) uint32_t bytes_s[MERGE ? 1 : NWIN];
+    __shared__ __align__(16) float lut_s[MERGE ? 1 : NLUT];
+    float2* const tw1 = MERGE ? aux : tw1_s;
+    float2* const tab = MERGE ? aux + NTW1L : tab_s;
+    uint32_t* const bytes = MERGE ? reinterpret_cast<uint32_t*>(aux + NTW1L + Gm::NTAB) : bytes_s;
+    float* const lut = MERGE ? reinterpret_cast<float*>(aux + NTW1L + Gm::NTAB) + NWIN : lut_s;
     const float2* rsub = tab;
     const float2* rstep = tab + SUB;
     const float2* frsub = tab + SUB + NSTEP;
@@ -606,7 +643,7 @@ void fx_fstage_kernel(FStageArgs args)
     const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
     int tid = threadIdx.x;
 
-    for (int e = tid; e < R0 * (R1 + 1); e += NT)
+    for (int e = tid; e < NTW1L; e += NT)
         tw1[e] = args.tw[((e / (R1 + 1)) * (e % (R1 + 1))) % NTW1 * (F / NTW1)];
     if constexpr (!CPLX) {
         if (tid < 32) {
@@ -701,7 +738,7 @@ void fx_fstage_kernel(FStageArgs args)
 #pragma unroll
             for (int w = 0; w < WPT; w++) {
                 const int k = tid + w * NT;
-                if (k < F / 8 + 4) bytes[k] = pol_codes(cur.w[w], pol);
+                if (k < NWIN) bytes[k] = pol_codes(cur.w[w], pol);
             }
         }
         __syncthreads();
@@ -727,11 +764,11 @@ void fx_fstage_kernel(FStageArgs args)
                 dft<R0>(x);
                 if constexpr (DIF) {
                     twiddle_pow<R0>(x, args.tw, i);
-                    float2* wr = buf + pad(i);
+                    float2* wr = buf + Gm::pad(i);
 #pragma unroll
-                    for (int r = 0; r < R0; r++) wr[r * (T0 + T0 / 16)] = x[r];
+                    for (int r = 0; r < R0; r++) wr[Gm::pad(r * T0)] = x[r];
                 } else {
-                    float2* wr = buf + pad(i * R0);
+                    float2* wr = buf + Gm::pad(i * R0);
 #pragma unroll
                     for (int r = 0; r < R0; r++) wr[r] = x[r];
                 }
@@ -767,11 +804,11 @@ void fx_fstage_kernel(FStageArgs args)
                 dft<R0>(x);
                 if constexpr (DIF) {
                     twiddle_pow<R0>(x, args.tw, i);
-                    float2* wr = buf + pad(i);
+                    float2* wr = buf + Gm::pad(i);
 #pragma unroll
-                    for (int r = 0; r < R0; r++) wr[r * (T0 + T0 / 16)] = x[r];
+                    for (int r = 0; r < R0; r++) wr[Gm::pad(r * T0)] = x[r];
                 } else {
-                    float2* wr = buf + pad(i * R0);
+                    float2* wr = buf + Gm::pad(i * R0);
 #pragma unroll
                     for (int r = 0; r < R0; r++) wr[r] = x[r];
                 }
@@ -787,9 +824,9 @@ void fx_fstage_kernel(FStageArgs args)
             const int k = (tid >> 6) * 32 + (tid & 31);
             constexpr int T2 = F / R2;
             float2 x[16];
-            const float2* rd = buf + pad(k + 256 * up);
+            const float2* rd = buf + Gm::pad(k + 256 * up);
 #pragma unroll
-            for (int r = 0; r < 16; r++) x[r] = rd[r * (T2 + T2 / 16)];
+            for (int r = 0; r < 16; r++) x[r] = rd[Gm::pad(r * T2)];
             twiddle_pow<16>(x, args.tw, k * (F / (256 * 16)));
             dft<16>(x);
             // x[r] is A[k + 256 r] (lanes 0-31) or B[k + 256 r] (lanes 32-63):
@@ -839,12 +876,12 @@ void fx_fstage_kernel(FStageArgs args)
 
         // ---- last pass: radix RL, the channels (real input: bins < nchan),
         // frac-sample rotation
-        // Stockham: butterfly i = tid + m*NT reads pad(i) + r*(TL + TL/16)
-        // = pad(tid) + m*(NT + NT/16) + ... and its channels c = i + r*PL have
+        // Stockham: butterfly i = tid + m*NT reads pad(i + r*TL)
+        // = pad(tid) + pad(m*NT) + pad(r*TL) and its channels c = i + r*PL have
         // c % STRIDE = tid % STRIDE and c / STRIDE = tid / STRIDE + (m*NT +
         // r*PL) / STRIDE, so every address is a per-thread base plus a constant
-        static_assert(NT % 16 == 0 && NT % STRIDE == 0 && PL % STRIDE == 0, "constant offsets");
-        const float2* rdb = buf + pad(tid);
+        static_assert(NT % Gm::PM == 0 && TL % Gm::PM == 0 && NT % STRIDE == 0 && PL % STRIDE == 0, "constant offsets");
+        const float2* rdb = buf + Gm::pad(tid);
         const float2 frs = frsub[tid & (STRIDE - 1)];
         const float2* frb = frstep + tid / STRIDE;
 #pragma unroll
@@ -854,15 +891,15 @@ void fx_fstage_kernel(FStageArgs args)
             int c0;
             if constexpr (DIF) {
                 // the RL consecutive points of butterfly i; no twiddle after the last DIF pass
-                static_assert(16 % RL == 0, "pad(i*RL + r) = pad(i*RL) + r");
-                const float2* rd = buf + pad(i * RL);
+                static_assert(Gm::PM % RL == 0, "pad(i*RL + r) = pad(i*RL) + r");
+                const float2* rd = buf + Gm::pad(i * RL);
 #pragma unroll
                 for (int r = 0; r < RL; r++) x[r] = rd[r];
                 c0 = dif_freq<Gm>(i);
             } else {
-                const float2* rd = rdb + m * (NT + NT / 16);
+                const float2* rd = rdb + Gm::pad(m * NT);
 #pragma unroll
-                for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
+                for (int r = 0; r < RL; r++) x[r] = rd[Gm::pad(r * TL)];
                 twiddle_pow<RL>(x, args.tw, i);
                 c0 = i;
             }
@@ -1044,7 +1081,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (2 * T0 / SUB)]);
                 }
                 dft<R0>(x);
-                float2* wr = buf + pad(i * R0);
+                float2* wr = buf + Gh::pad(i * R0);
 #pragma unroll
                 for (int r = 0; r < R0; r++) wr[r] = x[r];
             }
@@ -1056,9 +1093,9 @@ void fx_fstage_eo_kernel(FStageArgs args)
             for (int m = 0; m < PERL; m++) {
                 const int i = tid + m * NT;
                 float2 x[RL];
-                const float2* rd = buf + pad(i);
+                const float2* rd = buf + Gh::pad(i);
 #pragma unroll
-                for (int r = 0; r < RL; r++) x[r] = rd[r * (TL + TL / 16)];
+                for (int r = 0; r < RL; r++) x[r] = rd[Gh::pad(r * TL)];
                 twiddle_pow<RL>(x, args.tw, 2 * i);
                 dft<RL>(x);
                 if (h == 0) {
@@ -1138,7 +1175,7 @@ __global__ __launch_bounds__(Geo<kEoLG - 1>::NT) void fx_fft_c2c_eo_kernel(const
             for (int r = 0; r < R0; r++) x[r] = x_in[2 * (tid + r * T0) + h];
             dft<R0>(x);
 #pragma unroll
-            for (int r = 0; r < R0; r++) buf[pad(tid * R0 + r)] = x[r];
+            for (int r = 0; r < R0; r++) buf[Gh::pad(tid * R0 + r)] = x[r];
         }
         __syncthreads();
         mid_pass<Gh, R1, R0, true, 2>(buf, tw1, tw, tid);
@@ -1148,7 +1185,7 @@ __global__ __launch_bounds__(Geo<kEoLG - 1>::NT) void fx_fft_c2c_eo_kernel(const
             const int i = tid + m * NT;
             float2 x[RL];
 #pragma unroll
-            for (int r = 0; r < RL; r++) x[r] = buf[pad(i + r * TL)];
+            for (int r = 0; r < RL; r++) x[r] = buf[Gh::pad(i + r * TL)];
             twiddle_pow<RL>(x, tw, 2 * i);
             dft<RL>(x);
             if (h == 0) {
@@ -1198,7 +1235,7 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* _
         for (int r = 0; r < R0; r++) x[r] = x_in[i + r * T0];
         dft<R0>(x);
 #pragma unroll
-        for (int r = 0; r < R0; r++) buf[pad(i * R0 + r)] = x[r];
+        for (int r = 0; r < R0; r++) buf[Gm::pad(i * R0 + r)] = x[r];
     }
     __syncthreads();
     mid_pass<Gm, R1, R0, true>(buf, tw1, tw, tid);
@@ -1208,7 +1245,7 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* _
         const int i = tid + m * NT;
         float2 x[RL];
 #pragma unroll
-        for (int r = 0; r < RL; r++) x[r] = buf[pad(i + r * TL)];
+        for (int r = 0; r < RL; r++) x[r] = buf[Gm::pad(i + r * TL)];
         twiddle_pow<RL>(x, tw, i);
         dft<RL>(x);
 #pragma unroll
