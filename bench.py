@@ -225,15 +225,22 @@ def streaming_ingest(w, prec, data, delays, fileoff, nf, tuned, xbatch, nsub):
 
 def measured_traffic(workload, prec, xbatch, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{prec}_xbatch{xbatch}.md")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as f:
-        for line in f:
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                return int(parts[2]), os.path.relpath(path, ROOT)
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None.
+    An F-stage launch covers one subint whatever the number of subints per
+    X-engine launch, so for it a summary taken at any xbatch serves."""
+    pdir = os.path.join(ROOT, "profiles")
+    paths = [os.path.join(pdir, f"traffic_{workload}_{prec}_xbatch{xbatch}.md")]
+    if kernel == "fx_fstage_kernel" and os.path.isdir(pdir):
+        paths += sorted(os.path.join(pdir, n) for n in os.listdir(pdir)
+                        if n.startswith(f"traffic_{workload}_{prec}_xbatch") and n.endswith(".md"))
+    for path in paths:
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return int(parts[2]), os.path.relpath(path, ROOT)
     return None, None
 
 
