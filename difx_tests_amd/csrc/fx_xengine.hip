@@ -41,10 +41,11 @@ __device__ __forceinline__ uint4 conj_swap_f16(uint4 x)
     return make_uint4(f(x.x), f(x.y), f(x.z), f(x.w));
 }
 
-// (re, im) fp8 byte pairs -> (im, -re)
+// (re, im) fp8 byte pairs -> (im, -re): one byte permute (bytes 1 0 3 2) and
+// the sign flip of each pair's new high byte
 __device__ __forceinline__ uint32_t conj_swap_f8w(uint32_t w)
 {
-    return (((w >> 8) & 0x00ff00ffu) | ((w << 8) & 0xff00ff00u)) ^ 0x80008000u;
+    return __builtin_amdgcn_perm(w, w, 0x02030001u) ^ 0x80008000u;
 }
 
 __device__ __forceinline__ half8 as_half8(uint4 v)
@@ -108,6 +109,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     __shared__ __align__(16) uint4 lds[2][2][KC][PW];    // [stage][panel][group][signal]
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
+    // the launch has 64*WAVES threads: the panel of a load slot is then a
+    // compile-time function of the slot wherever whole waves share it
+    __builtin_assume(tid >= 0 && tid < 64 * WAVES);
     const int Spad = a.p.Spad, nblk = a.p.nblk;
     const int npan = (nblk + SB - 1) / SB;
     const int npairs = npan * (npan + 1) / 2;
@@ -157,8 +161,15 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const long long cstride = (long long)a.p.nchan * KC * Spad;
     const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * KC * Spad;
     const int nchunk = a.ngroups / KC;
-    // per-thread chunk loads: element e = (panel, group, signal), signal fastest
+    // per-thread chunk loads: element e = (panel, group, signal), signal
+    // fastest.  No divergent branch or zero fill per load: a signal past
+    // Spad = 32*nblk reads the last row instead (it only feeds blocks beyond
+    // nblk, which okm excludes), and a diagonal workgroup skips its second
+    // panel (a workgroup-uniform branch; that LDS half is not read).
+    static_assert(NELEM % (64 * WAVES) == 0, "whole loads per thread");
     uint4 st[NLD];
+#pragma unroll
+    for (int l = 0; l < NLD; l++) st[l] = make_uint4(0, 0, 0, 0);
     auto gload = [&](int chunk) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
@@ -166,20 +177,17 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
             const int g = rem / PW, sgl = rem - g * PW;
             const int sig = (pan ? SJ : SI) * PW + sgl;
-            st[l] = make_uint4(0, 0, 0, 0);
-            if (e < NELEM && sig < Spad && !(diag && pan))
-                st[l] = gbase[chunk * cstride + g * Spad + sig];
+            if (!(diag && pan))                           // uniform across the workgroup
+                st[l] = gbase[chunk * cstride + g * Spad + (sig < Spad ? sig : Spad - 1)];
         }
     };
     auto sstore = [&](int stage) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * WAVES;
-            if (e < NELEM) {
-                const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
-                const int g = rem / PW, sgl = rem - g * PW;
-                lds[stage][pan][g][sgl] = st[l];
-            }
+            const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+            const int g = rem / PW, sgl = rem - g * PW;
+            lds[stage][pan][g][sgl] = st[l];
         }
     };
 
