@@ -3,16 +3,23 @@
 //   fx_delay_kernel   FP64 delay model per (FFT, antenna): kernel.c:373-386
 //                     (getStationDelay) and kernel.c:736-756 (integer/
 //                     fractional split, edge clamps, validity).
-//   fx_fstage_kernel  fused 2-bit unpack (LDS byte window + 256-entry LDS level
-//                     table), fringe rotation (kernel.c:428-497), forward C2C FFT
-//                     (kernel.c:499-510) as LDS Stockham passes with in-register
-//                     radix-8/16/32 butterflies, fractional-sample correction of
-//                     channels < nchan (kernel.c:512-555) and the conversion to
-//                     the X-engine's corner-turned input (FP32 / FP16 / FP8).
+//   fx_rotator_kernel FP64-phase fringe and fractional-delay rotator tables
+//                     per (FFT, antenna) (kernel.c:428-497, 512-555).
+//   fx_fstage_kernel  fused 2-bit unpack (LDS byte window recoded to one
+//                     polarisation's code pairs + 32-word LDS level table),
+//                     fringe rotation (kernel.c:428-497), forward C2C FFT
+//                     (kernel.c:499-510) as LDS radix-8/16 passes with
+//                     in-register butterflies (Stockham, in-place
+//                     decimation in frequency, or a last radix-2 pass across
+//                     half-waves), fractional-sample correction of channels
+//                     < nchan (kernel.c:512-555) and the conversion to the
+//                     X-engine's corner-turned input (FP32 / FP16 / FP8).
+//   fx_fstage_eo_kernel  the same for F = 32768 from two 16384-point halves.
 //
-// One workgroup handles G consecutive FFTs of one antenna, both polarisations,
-// and keeps the G spectra of every channel it owns in registers, so each
-// (channel, signal) run of G FFTs leaves in one 16- or 32-byte store.
+// One workgroup handles G consecutive FFTs of one antenna and one
+// polarisation, and keeps the G spectra of every channel it owns in
+// registers, so each (channel, signal) run of G FFTs leaves in one 16-byte
+// store.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
@@ -287,11 +294,11 @@ __device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __rest
 // rotation, the last pass forms only channels < nchan and applies the
 // fractional-sample rotation.  A block transforms one polarisation and has
 // NT = F/R_max threads, so a radix-16 pass hands each thread exactly one
-// butterfly; at F = 8192 two blocks (79 KB of LDS each) share a CU, so one
+// butterfly; at F = 8192 two blocks (77-79 KB of LDS each) share a CU, so one
 // block's barriers and window loads overlap the other's arithmetic.
 
-// Plan V = 0 and, where a second one exists, V = 1; the autotuner picks per
-// context (fx_autotune).
+// Plans V = 0 .. nplans(LG) - 1; the autotuner picks per context
+// (fx_autotune).
 template <int LG, int V = 0> struct FPlan;
 template <> struct FPlan<9, 0>  { static constexpr int NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1, PADB = 4, KIND = 0; };
 template <> struct FPlan<10, 0> { static constexpr int NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1, PADB = 4, KIND = 0; };
@@ -1003,7 +1010,9 @@ void fx_fstage_eo_kernel(FStageArgs args)
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    __shared__ __align__(16) float lut[512];             // [byte][time slot], as in fx_fstage_kernel
+    // level of this pol at [byte][time slot]: the bytes come straight from
+    // global memory here, so they are not recoded to code pairs
+    __shared__ __align__(16) float lut[512];
     __shared__ __align__(16) float2 tab[NTAB];
     __shared__ float2 tw1[R0 * (R1 + 1)];
     const float2* rsub = tab;
