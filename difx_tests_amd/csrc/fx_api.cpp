@@ -64,6 +64,7 @@ struct fx_context {
     double* d_model;    // delays [nant*3] then fileoffsets [nant]
     int* d_iter;
     FxDelay* d_dtab;
+    uint8_t* d_valid;   // [max_ffts][nant] validity of each FFT slot's antennas (counts)
     float2* d_rtab;     // rotator tables [max_ffts][nant][sub + nstep + stride + nfstep]
     float2* d_tw;
     float* d_subchf;
@@ -111,7 +112,7 @@ static void free_ctx(fx_context* c)
         if (c->slot_free[k]) (void)hipEventDestroy(c->slot_free[k]);
     }
     if (c->cstream) (void)hipStreamDestroy(c->cstream);
-    void* bufs[] = {c->d_data, c->d_data1, c->d_model, c->d_iter, c->d_dtab, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
+    void* bufs[] = {c->d_data, c->d_data1, c->d_model, c->d_iter, c->d_dtab, c->d_valid, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
                     c->d_xin, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv,
                     c->d_cross, c->d_autos, c->d_tap};
     for (void* b : bufs)
@@ -217,6 +218,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     ALLOC(c->d_model, sizeof(double) * 4 * p.nant);
     ALLOC(c->d_iter, sizeof(int) * (c->max_ffts + 1));   // FFT indices, then the buffer origin
     ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
+    ALLOC(c->d_valid, (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * (p.sub + p.nstep + p.stride + p.nfstep));
     ALLOC(c->d_tw, sizeof(float2) * p.F);
     ALLOC(c->d_subchf, sizeof(float) * p.stride);
@@ -261,6 +263,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
         hipMemset(c->d_xin, 0, c->xin_bytes) != hipSuccess ||
         hipMemset(c->d_vacc, 0, c->vacc_bytes) != hipSuccess ||
         hipMemset(c->d_counts, 0, sizeof(int) * p.nant * p.nant) != hipSuccess ||
+        hipMemset(c->d_valid, 0, (size_t)c->max_ffts * p.nant) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         free_ctx(c);
         return fail(FX_EHIP, "context initialisation failed");
@@ -368,6 +371,7 @@ static int ev_collect(fx_context* c)
 // (never inside a graph capture).
 struct FStageCall {
     int numffts, npad, maxoffset, fft0, origin, slot;
+    int contig, it0;    // iteration[k] = it0 + k: nothing to upload
 };
 
 static int fstage_check(fx_context* c, int numffts, const int* iteration, int split, int fft0, FStageCall* f,
@@ -408,15 +412,22 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
 
 // FFT indices and the buffer origin to the device (ahead of the launches
 // that read them, on stream s).
-static int iter_upload(fx_context* c, const int* iteration, const FStageCall& f, hipStream_t s)
+// FFT indices and buffer origin for the delay kernel.  A contiguous run of
+// indices (the usual case) travels as kernel arguments; otherwise, or when
+// the launch is recorded into a graph that must read each subint's values
+// from device memory (device = true), they are copied to d_iter.
+static int iter_upload(fx_context* c, const int* iteration, FStageCall& f, hipStream_t s, bool device = false)
 {
+    f.it0 = iteration[0];
+    f.contig = !device;
+    for (int k = 1; k < f.numffts && f.contig; k++) f.contig = iteration[k] == f.it0 + k;
+    if (f.contig) return FX_OK;
     HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * f.numffts, hipMemcpyHostToDevice, s));
     c->h_origin = f.origin;
     HIPCHK(hipMemcpyAsync(c->d_iter + c->max_ffts, &c->h_origin, sizeof(int), hipMemcpyHostToDevice, s));
     return FX_OK;
 }
 
-// Delay table, F-stage and counts on stream s, reading c->d_iter.
 static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
 {
     const FxPlan& p = c->p;
@@ -424,9 +435,11 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     // read slot 0, filled by plain uploads)
     const bool ingest = c->ingest && !c->capturing;
     if (ingest) HIPCHK(hipStreamWaitEvent(s, c->copied[f.slot], 0));
-    TIMED(TM_DELAY, s, launch_delay(p, c->d_model, c->d_model + 3 * p.nant, c->d_iter, f.numffts, f.npad,
-                                    f.maxoffset, c->d_iter + c->max_ffts, c->d_dtab, s));
-    TIMED(TM_DELAY, s, launch_rotators(p, c->d_dtab, f.npad, c->d_subchf, c->d_stepchf, c->d_rtab, s));
+    TIMED(TM_DELAY, s, launch_delay_rotators(p, c->d_model, c->d_model + 3 * p.nant, f.contig ? nullptr : c->d_iter,
+                                             f.it0, f.numffts, f.npad, f.maxoffset,
+                                             f.contig ? nullptr : c->d_iter + c->max_ffts, f.origin, c->d_subchf,
+                                             c->d_stepchf, c->d_dtab, c->d_rtab,
+                                             c->d_valid + (size_t)f.fft0 * p.nant, s));
     FStageArgs a;
     a.p = p;
     a.data = f.slot ? c->d_data1 : c->d_data;
@@ -445,7 +458,6 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     a.plan = c->fplan;
     TIMED(TM_FSTAGE, s, launch_fstage(a, c->fprec, s));
     if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], s));           // the slot may be overwritten
-    TIMED(TM_COUNTS, s, launch_counts(p.nant, f.npad, c->d_dtab, c->d_counts, s));
     if (c->timing && !c->capturing) c->nl[0]++;
     c->last_nffts = f.npad;
     return FX_OK;
@@ -559,6 +571,8 @@ static int xstage_enqueue(fx_context* c, int fft0, int nslots, hipStream_t s)
     a.scale = c->d_scale;
     a.sb = c->xsb;
     TIMED(TM_XSTAGE, s, launch_xengine(a, c->prec, s));
+    // valid-FFT counts of the same slots (kernel.c:850)
+    TIMED(TM_COUNTS, s, launch_counts(c->p.nant, nslots, c->d_valid + (size_t)fft0 * c->p.nant, c->d_counts, s));
     if (c->timing && !c->capturing) c->nl[1]++;
     return FX_OK;
 }
@@ -947,7 +961,7 @@ extern "C" int fx_batch_process_ex(fx_batch* b, const int* numffts, const int* c
         HIPCHK(hipStreamWaitEvent(b->s0, b->pre[i], 0));
     }
     for (int i = 0; i < n; i++) {
-        int rc = iter_upload(b->ctx[i], iteration[i], calls[i], b->s0);
+        int rc = iter_upload(b->ctx[i], iteration[i], calls[i], b->s0, mode == FX_BATCH_GRAPH);
         if (rc) return rc;
     }
     if (mode == FX_BATCH_GRAPH) {

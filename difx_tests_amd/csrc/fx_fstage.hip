@@ -3,8 +3,9 @@
 //   fx_delay_kernel   FP64 delay model per (FFT, antenna): kernel.c:373-386
 //                     (getStationDelay) and kernel.c:736-756 (integer/
 //                     fractional split, edge clamps, validity).
-//   fx_rotator_kernel FP64-phase fringe and fractional-delay rotator tables
-//                     per (FFT, antenna) (kernel.c:428-497, 512-555).
+//   fx_delay_rot_kernel  the same delay entry fused with the FP64-phase
+//                     fringe and fractional-delay rotator tables per
+//                     (FFT, antenna) (kernel.c:428-497, 512-555).
 //   fx_fstage_kernel  fused 2-bit unpack (LDS byte window recoded to one
 //                     polarisation's code pairs + 32-word LDS level table),
 //                     fringe rotation (kernel.c:428-497), forward C2C FFT
@@ -33,24 +34,12 @@ static constexpr double kPi = 3.14159265358979323846;
 // --------------------------------------------------------------------------
 // FP64 delay table
 
-__global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
-                                const double* __restrict__ fileoff, const int* __restrict__ iteration,
-                                int numffts, int nffts_pad, int maxoffset, const int* __restrict__ origin_dev,
-                                FxDelay* __restrict__ out)
+// Delay-table entry of antenna j at global FFT index fftindex, for a buffer
+// whose first FFT is `origin` (kernel.c:373-386 and 736-756).
+__device__ FxDelay delay_entry(const FxPlan& p, const double* __restrict__ delays,
+                               const double* __restrict__ fileoff, int j, int fftindex, int origin, int maxoffset)
 {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nffts_pad * p.nant) return;
-    const int k = t / p.nant;
-    const int j = t - k * p.nant;
-    FxDelay e;
-    if (k >= numffts) {
-        e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
-        out[t] = e;
-        return;
-    }
-    const int fftindex = iteration[k];
-    const int origin = *origin_dev;
     const double c0 = delays[3 * j], c1 = delays[3 * j + 1], c2 = delays[3 * j + 2];
     // kernel.c:378-385
     const double d0 = c0 * fftindex * fftindex + c1 * fftindex + c2;
@@ -73,11 +62,30 @@ __global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
         --o;
         fd -= p.Ts;
     }
+    FxDelay e;
     e.a = a;
     e.b = b;
     e.fd = fd;
     e.offset = o;
     e.valid = !(o < 0 || o > maxoffset);
+    return e;
+}
+
+__global__ void fx_delay_kernel(FxPlan p, const double* __restrict__ delays,
+                                const double* __restrict__ fileoff, const int* __restrict__ iteration,
+                                int numffts, int nffts_pad, int maxoffset, const int* __restrict__ origin_dev,
+                                FxDelay* __restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nffts_pad * p.nant) return;
+    const int k = t / p.nant;
+    const int j = t - k * p.nant;
+    FxDelay e;
+    if (k >= numffts) {
+        e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
+    } else {
+        e = delay_entry(p, delays, fileoff, j, iteration[k], *origin_dev, maxoffset);
+    }
     out[t] = e;
 }
 
@@ -181,15 +189,14 @@ __device__ __forceinline__ float2 sincos_rounded(float xf)
 // FP64 phase arguments in the reference's operation order (kernel.c:428-497,
 // 512-555; contraction off), sin/cos rounded to FP32.
 
-__global__ void fx_rotator_kernel(FxPlan p, const FxDelay* __restrict__ dtab, const float* __restrict__ subchf,
-                                  const float* __restrict__ stepchf, float2* __restrict__ rtab)
+// Rotator tables of one (FFT, antenna) from its delay entry, block-strided:
+// sub fine and nstep step fringe rotators (FP64 phases, kernel.c:428-497),
+// stride fine and nfstep step fractional-delay rotators (kernel.c:512-555).
+__device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __restrict__ subchf,
+                             const float* __restrict__ stepchf, float2* __restrict__ out)
 {
 #pragma clang fp contract(off)
     const int ntab = p.sub + p.nstep + p.stride + p.nfstep;
-    const long long fa = blockIdx.x;                  // (FFT, antenna)
-    const FxDelay d = dtab[fa];
-    if (!d.valid) return;
-    float2* out = rtab + fa * ntab;
     const int integerdelay = (int)d.b;
     const double bfrac = d.b - integerdelay;
     const double lofr = p.lofrac * (double)integerdelay;
@@ -220,12 +227,45 @@ __global__ void fx_rotator_kernel(FxPlan p, const FxDelay* __restrict__ dtab, co
     }
 }
 
-hipError_t launch_rotators(const FxPlan& p, const FxDelay* dtab, int nffts, const float* subchf,
-                           const float* stepchf, float2* rtab, hipStream_t s)
+// One block per (FFT k of the call, antenna j): thread 0 forms the delay
+// entry (global FFT index iteration[k], or it0 + k when iteration is null;
+// buffer origin *origin_dev, or `origin` when that is null), writes it and
+// the FFT slot's validity byte, and the block fills the rotator tables.
+__global__ void fx_delay_rot_kernel(FxPlan p, const double* __restrict__ delays, const double* __restrict__ fileoff,
+                                    const int* __restrict__ iteration, int it0, int numffts, int maxoffset,
+                                    const int* __restrict__ origin_dev, int origin, const float* __restrict__ subchf,
+                                    const float* __restrict__ stepchf, FxDelay* __restrict__ dtab,
+                                    float2* __restrict__ rtab, uint8_t* __restrict__ valid_slots)
 {
-    const long long n = (long long)nffts * p.nant;
+    __shared__ FxDelay sd;
+    const long long fa = blockIdx.x;
+    const int k = (int)(fa / p.nant), j = (int)(fa - (long long)k * p.nant);
+    if (threadIdx.x == 0) {
+        FxDelay e;
+        if (k >= numffts) {
+            e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
+        } else {
+            const int g = iteration ? iteration[k] : it0 + k;
+            e = delay_entry(p, delays, fileoff, j, g, origin_dev ? *origin_dev : origin, maxoffset);
+        }
+        dtab[fa] = e;
+        valid_slots[fa] = (uint8_t)e.valid;
+        sd = e;
+    }
+    __syncthreads();
+    const FxDelay d = sd;
+    if (d.valid) rotator_fill(p, d, subchf, stepchf, rtab + fa * (p.sub + p.nstep + p.stride + p.nfstep));
+}
+
+hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
+                                 int it0, int numffts, int nffts_pad, int maxoffset, const int* origin_dev, int origin,
+                                 const float* subchf, const float* stepchf, FxDelay* dtab, float2* rtab,
+                                 uint8_t* valid_slots, hipStream_t s)
+{
+    const long long n = (long long)nffts_pad * p.nant;
     if (n <= 0) return hipSuccess;
-    fx_rotator_kernel<<<(unsigned)n, 128, 0, s>>>(p, dtab, subchf, stepchf, rtab);
+    fx_delay_rot_kernel<<<(unsigned)n, 128, 0, s>>>(p, delays, fileoff, iteration, it0, numffts, maxoffset,
+                                                     origin_dev, origin, subchf, stepchf, dtab, rtab, valid_slots);
     return hipGetLastError();
 }
 

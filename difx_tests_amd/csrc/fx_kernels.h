@@ -41,8 +41,14 @@ int fx_fstage_group_size(int prec);
 hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
                         int numffts, int nffts_pad, int maxoffset, const int* origin_dev, FxDelay* out,
                         hipStream_t s);
-hipError_t launch_rotators(const FxPlan& p, const FxDelay* dtab, int nffts, const float* subchf,
-                           const float* stepchf, float2* rtab, hipStream_t s);
+// Delay entries, FFT-slot validity bytes and rotator tables of one call in one
+// launch.  iteration: device FFT indices, or null for it0 + k; origin_dev:
+// device buffer origin, or null for `origin` (a recorded graph reads both from
+// device memory).  valid_slots: [nffts_pad][nant] bytes of the call's slots.
+hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
+                                 int it0, int numffts, int nffts_pad, int maxoffset, const int* origin_dev, int origin,
+                                 const float* subchf, const float* stepchf, FxDelay* dtab, float2* rtab,
+                                 uint8_t* valid_slots, hipStream_t s);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
 bool fstage_supported(int lg);
 bool fstage_supported_complex(int lg);   // complex 8-bit input, F = nchan
@@ -54,7 +60,9 @@ hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int l
 int xengine_chunk_groups(int prec);
 int xengine_super_blocks(int nblk);
 hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s);
-hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s);
+// counts[j*nant + k] (j <= k) += FFT slots of valid_slots[nslots][nant] where
+// both antennas are valid
+hipError_t launch_counts(int nant, int nslots, const uint8_t* valid_slots, int* counts, hipStream_t s);
 // counts (may be null): [nant*nant] valid-FFT counts; when given, the export
 // is normalised by them.
 hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, const int* counts,

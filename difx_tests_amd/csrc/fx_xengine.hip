@@ -463,30 +463,31 @@ hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s)
     }
 }
 
-// counts[j*nant + k] (j <= k) += number of FFTs where both antennas are valid
-// (kernel.c:850; j == k counts autocorrelation FFTs).
-// One thread per (antenna pair j <= k, chunk of 32 FFTs); the per-pair sums
-// of the chunks meet in counts[j*nant + k] through atomic adds.
-__global__ void fx_counts_kernel(int nant, int nffts, const FxDelay* __restrict__ d, int* __restrict__ counts)
+// counts[j*nant + k] (j <= k) += number of FFT slots where both antennas are
+// valid (kernel.c:850; j == k counts autocorrelation FFTs), over the slots one
+// X-engine launch reduces.  One thread per (antenna pair j <= k, run of 32
+// slots); the per-pair sums of the runs meet in counts[j*nant + k] through
+// atomic adds.
+__global__ void fx_counts_kernel(int nant, int nslots, const uint8_t* __restrict__ v, int* __restrict__ counts)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int pairs = nant * nant;
     const int chunk = (int)(t / pairs), jk = (int)(t - (long long)chunk * pairs);
     const int j = jk / nant, k = jk - j * nant;
     const int f0 = chunk * 32;
-    if (f0 >= nffts || j > k) return;
-    const int f1 = f0 + 32 < nffts ? f0 + 32 : nffts;
+    if (f0 >= nslots || j > k) return;
+    const int f1 = f0 + 32 < nslots ? f0 + 32 : nslots;
     int n = 0;
 #pragma unroll 8
-    for (int f = f0; f < f1; f++) n += d[(long long)f * nant + j].valid & d[(long long)f * nant + k].valid;
+    for (int f = f0; f < f1; f++) n += v[(long long)f * nant + j] & v[(long long)f * nant + k];
     if (n) atomicAdd(counts + jk, n);
 }
 
-hipError_t launch_counts(int nant, int nffts, const FxDelay* delays, int* counts, hipStream_t s)
+hipError_t launch_counts(int nant, int nslots, const uint8_t* valid_slots, int* counts, hipStream_t s)
 {
-    const long long n = (long long)nant * nant * ((nffts + 31) / 32);
+    const long long n = (long long)nant * nant * ((nslots + 31) / 32);
     if (n <= 0) return hipSuccess;
-    fx_counts_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nant, nffts, delays, counts);
+    fx_counts_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nant, nslots, valid_slots, counts);
     return hipGetLastError();
 }
 

@@ -121,6 +121,30 @@ def test_vis_fp32_matches_oracle(nchan, nant, nf, prec):
     assert _rel(autos[alive], ref["autos"][alive]).max() <= 1e-5
 
 
+@pytest.mark.parametrize("order", ["reversed", "every-other"])
+def test_vis_noncontiguous_iteration(order):
+    # FFT index lists that are not a contiguous run go to the device as an
+    # array (a contiguous run travels as kernel arguments): same visibilities
+    # and counts as the oracle on the same list
+    nchan, nant, nf = 1024, 5, 24
+    p, data, delays, fileoff, _ = _case(nchan, nant, 2 * nf, 77, spread=2.5)
+    it = np.arange(nf, dtype=np.int32)[::-1].copy() if order == "reversed" else np.arange(0, 2 * nf, 2, dtype=np.int32)
+    with _engine(p, nant, nf, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it, split=2)                     # maxoffset covers FFT 2*nf - 2
+        cross, cc, autos, ac = e.read_vis()
+        table = e.read_delay_table(nf)
+    ref = orc.process_f32(p, data, delays, fileoff, it, split=2)
+    assert np.array_equal(cc, ref["count"]) and np.array_equal(ac, ref["autocount"])
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= 1e-5
+    alive = ac > 0
+    assert _rel(autos[alive], ref["autos"][alive]).max() <= 1e-5
+    assert table["valid"].shape == (nf, nant) and table["valid"].sum() > nf
+
+
 def test_vis_fp32_split_and_accumulation():
     # two calls with split = 2 (kernel.c:730 maxoffset), accumulated on the device
     nchan, nant, nf = 512, 4, 8
