@@ -69,22 +69,27 @@ __device__ __forceinline__ void store_block(const XArgs& a, int c, int bp, int I
     float2* dst = a.vacc + ((long long)c * a.p.nbp + bp) * (FX_BLK * FX_BLK);
     float sn = 1.0f;
     if (sc) sn = sc[FX_BLK * J + n];
+    // in four quarters: 4 old values first, their loads in flight together,
+    // then the stores (more at once would cost a wave of occupancy)
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int m = (v >> 2) * 8 + kg * 4 + (v & 3);
-        float2 val = make_float2(re[v], im[v]);
-        if (sc) {
-            const float s = sc[FX_BLK * I + m] * sn;
-            val.x *= s;
-            val.y *= s;
+    for (int h = 0; h < 16; h += 4) {
+        float2 old[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int m = ((h + v) >> 2) * 8 + kg * 4 + ((h + v) & 3);
+            old[v] = a.accumulate ? dst[m * FX_BLK + n] : make_float2(0.0f, 0.0f);
         }
-        float2* e = dst + m * FX_BLK + n;
-        if (a.accumulate) {
-            const float2 o = *e;
-            val.x += o.x;
-            val.y += o.y;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int m = ((h + v) >> 2) * 8 + kg * 4 + ((h + v) & 3);
+            float2 val = make_float2(re[h + v], im[h + v]);
+            if (sc) {
+                const float s = sc[FX_BLK * I + m] * sn;
+                val.x *= s;
+                val.y *= s;
+            }
+            dst[m * FX_BLK + n] = make_float2(val.x + old[v].x, val.y + old[v].y);
         }
-        *e = val;
     }
 }
 
@@ -97,16 +102,18 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // One workgroup = one channel x one super-tile pair (SI, SJ) of SB*32-signal
 // panels.  The pair's 32x32 blocks are dealt round-robin to the WAVES waves
 // (at most NBW each).  K chunks of both panels are staged through LDS by all
-// threads with coalesced 16-byte loads, double-buffered: the global loads of
-// chunk k+1 are in flight while the MFMAs of chunk k run.
-template <int PREC, int SB>
+// threads with coalesced 16-byte loads, double-buffered in steps of NSUB
+// chunks: the global loads of step k+1 are in flight while the MFMAs of step
+// k run.  NSUB > 1 serves grids too small to fill the GPU (few channels and
+// blocks), whose workgroups would otherwise wait one memory latency per chunk.
+template <int PREC, int SB, int NSUB>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
     constexpr int PW = SB * FX_BLK;                      // signals per panel
     constexpr int NELEM = 2 * PW * KC;                   // 16-byte rows per chunk (both panels)
-    constexpr int NLD = (NELEM + 64 * WAVES - 1) / (64 * WAVES);
-    __shared__ __align__(16) uint4 lds[2][2][KC][PW];    // [stage][panel][group][signal]
+    constexpr int NLD = (NSUB * NELEM + 64 * WAVES - 1) / (64 * WAVES);
+    __shared__ __align__(16) uint4 lds[2][NSUB][2][KC][PW];   // [stage][chunk][panel][group][signal]
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
     // the launch has 64*WAVES threads: the panel of a load slot is then a
@@ -161,33 +168,42 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const long long cstride = (long long)a.p.nchan * KC * Spad;
     const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * KC * Spad;
     const int nchunk = a.ngroups / KC;
-    // per-thread chunk loads: element e = (panel, group, signal), signal
-    // fastest.  No divergent branch or zero fill per load: a signal past
-    // Spad = 32*nblk reads the last row instead (it only feeds blocks beyond
-    // nblk, which okm excludes), and a diagonal workgroup skips its second
-    // panel (a workgroup-uniform branch; that LDS half is not read).
+    const int nstep = (nchunk + NSUB - 1) / NSUB;
+    // per-thread loads of step sp: element e = (chunk in step, panel, group,
+    // signal), signal fastest.  No divergent branch or zero fill per load: a
+    // signal past Spad = 32*nblk reads the last row instead (it only feeds
+    // blocks beyond nblk, which okm excludes), and with one chunk per step a
+    // diagonal workgroup skips its second panel (a workgroup- or wave-uniform
+    // branch; that LDS is not read).
     static_assert(NELEM % (64 * WAVES) == 0, "whole loads per thread");
     uint4 st[NLD];
 #pragma unroll
     for (int l = 0; l < NLD; l++) st[l] = make_uint4(0, 0, 0, 0);
-    auto gload = [&](int chunk) {
+    auto gload = [&](int sp) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * WAVES;
-            const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+            const int sub = e / NELEM, e2 = e - sub * NELEM;
+            const int pan = e2 / (KC * PW), rem = e2 - pan * (KC * PW);
             const int g = rem / PW, sgl = rem - g * PW;
             const int sig = (pan ? SJ : SI) * PW + sgl;
-            if (!(diag && pan))                           // uniform across the workgroup
-                st[l] = gbase[chunk * cstride + g * Spad + (sig < Spad ? sig : Spad - 1)];
+            // the last step's chunks past nchunk re-read the last chunk; with
+            // several chunks per step every load is issued, so that they go
+            // out back to back (a branch between them serialises them)
+            const int chunk = sp * NSUB + sub;
+            const int chc = chunk < nchunk ? chunk : nchunk - 1;
+            if (NSUB > 1 || !(diag && pan))
+                st[l] = gbase[chc * cstride + g * Spad + (sig < Spad ? sig : Spad - 1)];
         }
     };
     auto sstore = [&](int stage) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * WAVES;
-            const int pan = e / (KC * PW), rem = e - pan * (KC * PW);
+            const int sub = e / NELEM, e2 = e - sub * NELEM;
+            const int pan = e2 / (KC * PW), rem = e2 - pan * (KC * PW);
             const int g = rem / PW, sgl = rem - g * PW;
-            lds[stage][pan][g][sgl] = st[l];
+            lds[stage][sub][pan][g][sgl] = st[l];
         }
     };
 
@@ -200,22 +216,25 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const int row = lane & 31, kg = lane >> 5;
     const int pb = diag ? 0 : 1;                          // panel of the B operand
 
-    if (nchunk > 0) {
+    if (nstep > 0) {
         gload(0);
         sstore(0);
     }
     __syncthreads();
-    for (int ch = 0; ch < nchunk; ch++) {
-        const int stg = ch & 1;
-        if (ch + 1 < nchunk) gload(ch + 1);
+    for (int sp = 0; sp < nstep; sp++) {
+        const int stg = sp & 1;
+        if (sp + 1 < nstep) gload(sp + 1);
+#pragma unroll
+        for (int sub = 0; sub < NSUB; sub++) {
+        if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
         if constexpr (PREC == FX_PREC_F16) {
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
-                        const uint4 A = lds[stg][0][2 * ks + kg][bI[t] * FX_BLK + row];
-                        const uint4 B = lds[stg][pb][2 * ks + kg][bJ[t] * FX_BLK + row];
+                        const uint4 A = lds[stg][sub][0][2 * ks + kg][bI[t] * FX_BLK + row];
+                        const uint4 B = lds[stg][sub][pb][2 * ks + kg][bJ[t] * FX_BLK + row];
                         const half8 hb = as_half8(B);
                         accr[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(A), hb, accr[t], 0, 0, 0);
                         acci[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(conj_swap_f16(A)), hb, acci[t], 0,
@@ -230,8 +249,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
-                        const float4 A = as_float4(lds[stg][0][g][bI[t] * FX_BLK + row]);
-                        const float4 B = as_float4(lds[stg][pb][g][bJ[t] * FX_BLK + row]);
+                        const float4 A = as_float4(lds[stg][sub][0][g][bI[t] * FX_BLK + row]);
+                        const float4 B = as_float4(lds[stg][sub][pb][g][bJ[t] * FX_BLK + row]);
                         const float av[2] = {kg ? A.z : A.x, kg ? A.w : A.y};
                         const float cv[2] = {kg ? -A.x : A.z, kg ? -A.y : A.w};
                         const float bv[2] = {kg ? B.z : B.x, kg ? B.w : B.y};
@@ -250,10 +269,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
-                        const uint4 a0 = lds[stg][0][4 * ks + 2 * kg][bI[t] * FX_BLK + row];
-                        const uint4 a1 = lds[stg][0][4 * ks + 2 * kg + 1][bI[t] * FX_BLK + row];
-                        const uint4 b0 = lds[stg][pb][4 * ks + 2 * kg][bJ[t] * FX_BLK + row];
-                        const uint4 b1 = lds[stg][pb][4 * ks + 2 * kg + 1][bJ[t] * FX_BLK + row];
+                        const uint4 a0 = lds[stg][sub][0][4 * ks + 2 * kg][bI[t] * FX_BLK + row];
+                        const uint4 a1 = lds[stg][sub][0][4 * ks + 2 * kg + 1][bI[t] * FX_BLK + row];
+                        const uint4 b0 = lds[stg][sub][pb][4 * ks + 2 * kg][bJ[t] * FX_BLK + row];
+                        const uint4 b1 = lds[stg][sub][pb][4 * ks + 2 * kg + 1][bJ[t] * FX_BLK + row];
                         const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
                                           (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
                         i32x8 cv;
@@ -267,7 +286,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
                 }
             }
         }
-        if (ch + 1 < nchunk) sstore(stg ^ 1);
+        }   // sub
+        if (sp + 1 < nstep) sstore(stg ^ 1);
         __syncthreads();
     }
 
@@ -421,6 +441,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 
 int xengine_super_blocks(int nblk) { return nblk >= 4 ? 4 : nblk >= 2 ? 2 : 1; }
 
+// Grids of at most this many one-block-tile workgroups stage several K chunks
+// per step: at four per CU (32 KB of LDS each) they still run in one round,
+// and each workgroup keeps four chunks in flight instead of one (larger grids
+// lose more to a second round than they gain).
+static constexpr int kSmallGrid = 1024;
+
 template <int PREC>
 static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
 {
@@ -428,10 +454,14 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
     const int npan = (a.p.nblk + SB - 1) / SB;
     const int npairs = npan * (npan + 1) / 2;
     const dim3 grid(a.p.nchan * npairs), block(64 * WAVES);
+    constexpr int DEEP = PREC == FX_PREC_F8 ? 2 : 4;    // 32 KB of LDS either way
     switch (SB) {
-    case 4: fx_xengine_kernel<PREC, 4><<<grid, block, 0, s>>>(a); break;
-    case 2: fx_xengine_kernel<PREC, 2><<<grid, block, 0, s>>>(a); break;
-    default: fx_xengine_kernel<PREC, 1><<<grid, block, 0, s>>>(a); break;
+    case 4: fx_xengine_kernel<PREC, 4, 1><<<grid, block, 0, s>>>(a); break;
+    case 2: fx_xengine_kernel<PREC, 2, 1><<<grid, block, 0, s>>>(a); break;
+    default:
+        if ((long long)a.p.nchan * npairs <= kSmallGrid) fx_xengine_kernel<PREC, 1, DEEP><<<grid, block, 0, s>>>(a);
+        else fx_xengine_kernel<PREC, 1, 1><<<grid, block, 0, s>>>(a);
+        break;
     }
     return hipGetLastError();
 }
