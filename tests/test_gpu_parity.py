@@ -244,6 +244,32 @@ def test_vis_multi_step_k_pipeline(prec, nf, xtile):
     assert _rel(autos[alive], ref["autos"][alive]).max() <= tol
 
 
+@pytest.mark.parametrize("prec,nf", [("f32", 40), ("f16", 80), ("f16", 100), ("f8", 192)])
+@pytest.mark.parametrize("nchan", [256, 1024])
+def test_vis_small_grid_k_pipeline(prec, nf, nchan):
+    # one 32-signal block per channel and at most 1024 workgroups: the X-engine
+    # stages several K chunks per step; chunk counts 5, 7 and 3 leave the last
+    # step partly empty
+    nant = 8
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 400 + nf + nchan, data="correlated")
+    with _engine(p, nant, nf, prec=prec, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    if prec == "f32":
+        ref, tol, tola = orc.process_f32(p, data, delays, fileoff, it), 1e-5, 1e-5
+    else:
+        ref = orc.process_f64(p, data, delays, fileoff, it)
+        tol, tola = (1e-3, 1e-3) if prec == "f16" else (0.1, 0.1)
+    assert np.array_equal(cc, ref["count"]) and np.array_equal(ac, ref["autocount"])
+    live = cc > 0
+    assert _rel(cross[live], ref["vis"][live]).max() <= tol
+    alive = ac > 0
+    assert _rel(autos[alive], ref["autos"][alive]).max() <= tola
+
+
 @pytest.mark.parametrize("nchan,nant,nf", [(1024, 4, 64), (2048, 12, 32), (16384, 4, 24)])
 def test_vis_fp8_vs_fp64_oracle(nchan, nant, nf):
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 50 + nchan, data="correlated")
