@@ -219,7 +219,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     ALLOC(c->d_iter, sizeof(int) * (c->max_ffts + 1));   // FFT indices, then the buffer origin
     ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_valid, (size_t)c->max_ffts * p.nant);
-    ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * (p.sub + p.nstep + p.stride + p.nfstep));
+    ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * fx_rtab_len(p));
     ALLOC(c->d_tw, sizeof(float2) * p.F);
     ALLOC(c->d_subchf, sizeof(float) * p.stride);
     ALLOC(c->d_stepchf, sizeof(float) * p.nfstep);
@@ -438,8 +438,8 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     TIMED(TM_DELAY, s, launch_delay_rotators(p, c->d_model, c->d_model + 3 * p.nant, f.contig ? nullptr : c->d_iter,
                                              f.it0, f.numffts, f.npad, f.maxoffset,
                                              f.contig ? nullptr : c->d_iter + c->max_ffts, f.origin, c->d_subchf,
-                                             c->d_stepchf, c->d_dtab, c->d_rtab,
-                                             c->d_valid + (size_t)f.fft0 * p.nant, s));
+                                             c->d_stepchf, fstage_step_q(p.lg, c->fplan, p.iscomplex), c->d_dtab,
+                                             c->d_rtab, c->d_valid + (size_t)f.fft0 * p.nant, s));
     FStageArgs a;
     a.p = p;
     a.data = f.slot ? c->d_data1 : c->d_data;

@@ -42,6 +42,15 @@ struct FxPlan {
     double Ts, lo, bw, lofrac;       // lofrac = lo - (int)lo
 };
 
+// Rotator tables of one (FFT, antenna): fringe sub [sub], fringe step
+// [nstep], fractional sub [stride], fractional step [nfstep], then the fringe
+// steps again in the order the F-stage's scalar loads read them
+// (step b + q*r at b*(nstep/q) + r, q set by the radix plan).
+static inline __host__ __device__ int fx_rtab_len(const FxPlan& p)
+{
+    return p.sub + 2 * p.nstep + p.stride + p.nfstep;
+}
+
 // Upper-triangle block-pair index of (I, J), I <= J < nblk, row-major.
 static inline __host__ __device__ int fx_bp_index(int I, int J, int nblk)
 {

@@ -193,10 +193,13 @@ __device__ __forceinline__ float2 sincos_rounded(float xf)
 // sub fine and nstep step fringe rotators (FP64 phases, kernel.c:428-497),
 // stride fine and nfstep step fractional-delay rotators (kernel.c:512-555).
 __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __restrict__ subchf,
-                             const float* __restrict__ stepchf, float2* __restrict__ out)
+                             const float* __restrict__ stepchf, int step_q, float2* __restrict__ out)
 {
 #pragma clang fp contract(off)
     const int ntab = p.sub + p.nstep + p.stride + p.nfstep;
+    // step_q and nstep are powers of two (checked at launch)
+    const int qbits = step_q > 0 ? __builtin_ctz(step_q) : 0;
+    const int rbits = __builtin_ctz(p.nstep) - qbits;
     const int integerdelay = (int)d.b;
     const double bfrac = d.b - integerdelay;
     const double lofr = p.lofrac * (double)integerdelay;
@@ -223,7 +226,12 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
             arg = stepchf[k] * fd32;
             unit = (k == 0);
         }
-        out[e] = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
+        const float2 v = unit ? make_float2(1.0f, 0.0f) : sincos_rounded(arg);
+        out[e] = v;
+        if (step_q > 0 && e >= p.sub && e < p.sub + p.nstep) {
+            const int k = e - p.sub;
+            out[ntab + ((k & (step_q - 1)) << rbits) + (k >> qbits)] = v;
+        }
     }
 }
 
@@ -234,7 +242,7 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
 __global__ void fx_delay_rot_kernel(FxPlan p, const double* __restrict__ delays, const double* __restrict__ fileoff,
                                     const int* __restrict__ iteration, int it0, int numffts, int maxoffset,
                                     const int* __restrict__ origin_dev, int origin, const float* __restrict__ subchf,
-                                    const float* __restrict__ stepchf, FxDelay* __restrict__ dtab,
+                                    const float* __restrict__ stepchf, int step_q, FxDelay* __restrict__ dtab,
                                     float2* __restrict__ rtab, uint8_t* __restrict__ valid_slots)
 {
     __shared__ FxDelay sd;
@@ -254,18 +262,21 @@ __global__ void fx_delay_rot_kernel(FxPlan p, const double* __restrict__ delays,
     }
     __syncthreads();
     const FxDelay d = sd;
-    if (d.valid) rotator_fill(p, d, subchf, stepchf, rtab + fa * (p.sub + p.nstep + p.stride + p.nfstep));
+    if (d.valid) rotator_fill(p, d, subchf, stepchf, step_q, rtab + fa * fx_rtab_len(p));
 }
 
 hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
                                  int it0, int numffts, int nffts_pad, int maxoffset, const int* origin_dev, int origin,
-                                 const float* subchf, const float* stepchf, FxDelay* dtab, float2* rtab,
+                                 const float* subchf, const float* stepchf, int step_q, FxDelay* dtab, float2* rtab,
                                  uint8_t* valid_slots, hipStream_t s)
 {
     const long long n = (long long)nffts_pad * p.nant;
     if (n <= 0) return hipSuccess;
+    if (step_q < 0 || (step_q & (step_q - 1)) != 0 || (p.nstep & (p.nstep - 1)) != 0 || step_q > p.nstep)
+        return hipErrorInvalidValue;
     fx_delay_rot_kernel<<<(unsigned)n, 128, 0, s>>>(p, delays, fileoff, iteration, it0, numffts, maxoffset,
-                                                     origin_dev, origin, subchf, stepchf, dtab, rtab, valid_slots);
+                                                     origin_dev, origin, subchf, stepchf, step_q, dtab, rtab,
+                                                     valid_slots);
     return hipGetLastError();
 }
 
@@ -380,6 +391,7 @@ template <int LG, int V = 0, bool CPLX = false> struct Geo {
     static constexpr int NSTEP = F / SUB;
     static constexpr int NFSTEP = NCH / STRIDE;
     static constexpr int NTAB = SUB + NSTEP + STRIDE + NFSTEP;
+    static constexpr int RTAB = NTAB + NSTEP;                // table stride in memory (fx_rtab_len)
     static constexpr int NP = PL_::NP, R0 = PL_::R0, R1 = PL_::R1, R2 = PL_::R2, R3 = PL_::R3;
     static constexpr int RMAX = cmax(cmax(R0, R1), cmax(R2, R3));
     static constexpr int NT = F / RMAX;
@@ -425,6 +437,19 @@ __device__ __forceinline__ float2 load_const(const float2* p)
     return make_float2(q[0], q[1]);
 #else
     return *p;
+#endif
+}
+
+// (offset, valid) of a delay-table entry through the constant address space
+// (scalar loads at a block-uniform address).
+__device__ __forceinline__ int2 load_delay_ov(const FxDelay* d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const int cint;
+    const cint* q = (const cint*)&d->offset;
+    return make_int2(q[0], q[1]);
+#else
+    return make_int2(d->offset, d->valid);
 #endif
 }
 
@@ -618,6 +643,48 @@ __host__ __device__ constexpr int fstage_wpe()
     return (LG == 12 && !CPLX && PREC != FX_PREC_F8) ? 4 : Geo<LG, V, CPLX>::WPE;
 }
 
+// Pass 0 of real input reads its fringe steps with scalar loads when the
+// step index is uniform across each wave; with one pass-0 butterfly per
+// thread it reads them from the tables' second copy of the steps, as one
+// contiguous block.
+template <int LG, int V, bool CPLX>
+__host__ __device__ constexpr bool fstage_sstep()
+{
+    using Gm = Geo<LG, V, CPLX>;
+    return !CPLX && (Gm::SUB % 64 == 0) && (Gm::NT % Gm::SUB == 0);
+}
+template <int LG, int V, bool CPLX>
+__host__ __device__ constexpr bool fstage_sperm()
+{
+    using Gm = Geo<LG, V, CPLX>;
+    return fstage_sstep<LG, V, CPLX>() && Gm::F / Gm::R0 == Gm::NT;
+}
+
+template <int LG, bool CPLX>
+static int step_q_lg(int plan)
+{
+    using G0 = Geo<LG, 0, CPLX>;
+    if constexpr (nplans(LG) > 2)
+        if (plan == 2) return fstage_sperm<LG, 2, CPLX>() ? G0::F / Geo<LG, 2, CPLX>::R0 / G0::SUB : 0;
+    if constexpr (nplans(LG) > 1)
+        if (plan == 1) return fstage_sperm<LG, 1, CPLX>() ? G0::F / Geo<LG, 1, CPLX>::R0 / G0::SUB : 0;
+    return fstage_sperm<LG, 0, CPLX>() ? G0::F / G0::R0 / G0::SUB : 0;
+}
+
+int fstage_step_q(int lg, int plan, int iscomplex)
+{
+    if (iscomplex) return 0;
+    switch (lg) {
+    case 9:  return step_q_lg<9, false>(plan);
+    case 10: return step_q_lg<10, false>(plan);
+    case 11: return step_q_lg<11, false>(plan);
+    case 12: return step_q_lg<12, false>(plan);
+    case 13: return step_q_lg<13, false>(plan);
+    case 14: return step_q_lg<14, false>(plan);
+    default: return 0;                        // the 32768-point halves read the steps from the LDS
+    }
+}
+
 template <int LG, int V, int PREC, bool TAP, bool CPLX = false>
 __global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_eu(fstage_wpe<LG, V, PREC, CPLX>())))
 void fx_fstage_kernel(FStageArgs args)
@@ -704,19 +771,22 @@ void fx_fstage_kernel(FStageArgs args)
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
     OutRegs<PREC, G, NOUT> outr;
 
-    // The next FFT's offset, validity, rotator tables and packed-byte window
-    // are loaded into registers while the current FFT runs.
+    // The next FFT's rotator tables and packed-byte window are loaded into
+    // registers while the current FFT runs; the delay entry (offset, valid)
+    // that their addresses need is loaded one FFT earlier still, so no FFT
+    // waits for a load issued after its predecessor's barrier.
     struct Prefetch {
         int valid, offset;
         float2 t[TPT];
         uint32_t w[WPT];
     };
-    auto fetch = [&](int gi, Prefetch& f) {
+    auto dload = [&](int gi) { return load_delay_ov(args.delays + (long long)(grp * G + gi) * p.nant + ant); };
+    auto fetch = [&](int gi, Prefetch& f, int2 d) {
         const long long fa = (long long)(grp * G + gi) * p.nant + ant;
-        f.valid = args.delays[fa].valid;
-        f.offset = args.delays[fa].offset;
+        f.offset = d.x;
+        f.valid = d.y;
         if (!f.valid) return;
-        const float2* tb = args.rtab + fa * Gm::NTAB;
+        const float2* tb = args.rtab + fa * Gm::RTAB;
 #pragma unroll
         for (int t = 0; t < TPT; t++) {
             const int e = threadIdx.x + t * NT;
@@ -734,7 +804,8 @@ void fx_fstage_kernel(FStageArgs args)
         }
     };
     Prefetch cur;
-    fetch(0, cur);
+    fetch(0, cur, dload(0));
+    int2 dnext = dload(G > 1 ? 1 : 0);
 
     // XL: lanes 0-31 of wave w own the third-pass butterflies k = 32w + lane,
     // lanes 32-63 the butterflies 256 + k; their outputs pair up in the last
@@ -770,7 +841,10 @@ void fx_fstage_kernel(FStageArgs args)
                         args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + tid + m * NT + r * PL] =
                             make_float2(0.0f, 0.0f);
             }
-            if (gi + 1 < G) fetch(gi + 1, cur);
+            if (gi + 1 < G) {
+                fetch(gi + 1, cur, dnext);
+                if (gi + 2 < G) dnext = dload(gi + 2);
+            }
             continue;
         }
         // ---- tables and window of this FFT from registers to LDS
@@ -789,7 +863,10 @@ void fx_fstage_kernel(FStageArgs args)
             }
         }
         __syncthreads();
-        if (gi + 1 < G) fetch(gi + 1, cur);
+        if (gi + 1 < G) {
+            fetch(gi + 1, cur, dnext);
+            if (gi + 2 < G) dnext = dload(gi + 2);
+        }
 
         // ---- pass 0: unpack + fringe rotate + radix-R0 butterfly
         if constexpr (CPLX) {
@@ -823,10 +900,15 @@ void fx_fstage_kernel(FStageArgs args)
         } else {
             const uint8_t* bl = reinterpret_cast<const uint8_t*>(bytes);
             // SUB a multiple of the wave size: the step rotator index i / SUB is
-            // the same in every lane of a wave, so it is read with scalar loads
-            // from the rotator table in L2 instead of the LDS
-            constexpr bool SSTEP = (SUB % 64 == 0) && (NT % SUB == 0);
-            const float2* rstep_g = args.rtab + ((long long)fft * p.nant + ant) * Gm::NTAB + SUB;
+            // the same in every lane of a wave, so the R0 steps b + Q r of a
+            // thread (b = i / SUB, Q = T0 / SUB) are read with scalar loads from
+            // the tables in L2 instead of the LDS: with one butterfly per thread
+            // (SPERM) from the tables' second copy of the steps, where they are
+            // contiguous, all issued before one wait; otherwise from the steps
+            constexpr bool SSTEP = fstage_sstep<LG, V, CPLX>(), SPERM = fstage_sperm<LG, V, CPLX>();
+            constexpr int Q = T0 / SUB;
+            const float2* rtab_fa = args.rtab + ((long long)fft * p.nant + ant) * Gm::RTAB;
+            const float2* rstep_g = rtab_fa + (SPERM ? Gm::NTAB : SUB);
 #pragma unroll
             for (int m = 0; m < PER0; m++) {
                 const int i = tid + m * NT;
@@ -838,14 +920,22 @@ void fx_fstage_kernel(FStageArgs args)
                 const float* lt = lut + (q0 & 1);
                 const uint8_t* bp = bl + ((q0 >> 1) - bbase);
                 const float2* sp = rstep + i / SUB;
-                const float2* spg = rstep_g + __builtin_amdgcn_readfirstlane(i / SUB);
+                const float2* spg = rstep_g + __builtin_amdgcn_readfirstlane(i / SUB) * (SPERM ? R0 : 1);
                 float2 x[R0];
+                [[maybe_unused]] float2 stv[R0];
+                if constexpr (SPERM) {
+#pragma unroll
+                    for (int r = 0; r < R0; r++) stv[r] = load_const(spg + r);
+                    // the loads complete here, before the LDS reads below,
+                    // whose waits then count LDS operations only
+                    asm volatile("" ::"s"(stv[R0 - 1].x));
+                }
 #pragma unroll
                 for (int r = 0; r < R0; r++) {
                     // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
                     // fine rotator first, then one contracted complex product
                     const float l = lt[2 * bp[r * (T0 / 2)]];
-                    const float2 st = SSTEP ? load_const(spg + r * (T0 / SUB)) : sp[r * (T0 / SUB)];
+                    const float2 st = SPERM ? stv[r] : SSTEP ? load_const(spg + r * Q) : sp[r * Q];
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), st);
                 }
                 dft<R0>(x);
@@ -1104,7 +1194,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
             continue;
         }
         {
-            const float2* tb = args.rtab + ((long long)fft * p.nant + ant) * NTAB;
+            const float2* tb = args.rtab + ((long long)fft * p.nant + ant) * (NTAB + NSTEP);
             for (int e = tid; e < NTAB; e += NT) tab[e] = tb[e];
         }
         __syncthreads();

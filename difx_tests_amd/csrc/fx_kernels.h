@@ -45,10 +45,16 @@ hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fil
 // launch.  iteration: device FFT indices, or null for it0 + k; origin_dev:
 // device buffer origin, or null for `origin` (a recorded graph reads both from
 // device memory).  valid_slots: [nffts_pad][nant] bytes of the call's slots.
+// step_q: order of the tables' second copy of the fringe steps
+// (fstage_step_q; 0 leaves it unwritten).
 hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
                                  int it0, int numffts, int nffts_pad, int maxoffset, const int* origin_dev, int origin,
-                                 const float* subchf, const float* stepchf, FxDelay* dtab, float2* rtab,
+                                 const float* subchf, const float* stepchf, int step_q, FxDelay* dtab, float2* rtab,
                                  uint8_t* valid_slots, hipStream_t s);
+// The F-stage kernel of radix plan `plan` reads its pass-0 fringe steps b + q*r
+// (r < R0) with scalar loads from the tables' second copy of the steps; returns
+// that q, or 0 when the plan reads them from the LDS.
+int fstage_step_q(int lg, int plan, int iscomplex);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
 bool fstage_supported(int lg);
 bool fstage_supported_complex(int lg);   // complex 8-bit input, F = nchan
