@@ -316,28 +316,46 @@ __device__ __forceinline__ void dft(float2 (&x)[R])
     for (int i = 0; i < R; i++) x[i] = y[i];
 }
 
-// x[r] *= W^(r*e) for r = 1..R-1, W = exp(-2 pi i / F) from the table tw.
-// Each twiddle is one product of two loaded factors, W^(4ae) * W^(be) for
-// r = 4a + b, so a radix-R butterfly loads R/4 + 3 table entries.
-template <int R>
-__device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __restrict__ tw, int e)
-{
+// Factors of the twiddles W^(r*e), r < R, W = exp(-2 pi i / F) from the
+// table tw: each twiddle is one product of two loaded factors, W^(4ae) *
+// W^(be) for r = 4a + b, so a radix-R butterfly loads R/4 + 3 table entries.
+template <int R> struct TwFac {
     float2 lo[4], hi[(R / 4) > 0 ? R / 4 : 1];
-    lo[0] = make_float2(1.0f, 0.0f);
+};
+
+template <int R>
+__device__ __forceinline__ void twiddle_fac(TwFac<R>& f, const float2* __restrict__ tw, int e)
+{
+    f.lo[0] = make_float2(1.0f, 0.0f);
 #pragma unroll
-    for (int b = 1; b < 4 && b < R; b++) lo[b] = tw[b * e];
-    hi[0] = make_float2(1.0f, 0.0f);
+    for (int b = 1; b < 4 && b < R; b++) f.lo[b] = tw[b * e];
+    f.hi[0] = make_float2(1.0f, 0.0f);
 #pragma unroll
-    for (int a = 1; a < R / 4; a++) hi[a] = tw[4 * a * e];
+    for (int a = 1; a < R / 4; a++) f.hi[a] = tw[4 * a * e];
+}
+
+// x[r] *= W^(r*e) for r = 1..R-1 from the factors
+template <int R>
+__device__ __forceinline__ void twiddle_apply(float2 (&x)[R], const TwFac<R>& f)
+{
 #pragma unroll
     for (int r = 1; r < R; r++) {
         const int a = r >> 2, b = r & 3;
         float2 w;
-        if (a == 0) w = lo[b];
-        else if (b == 0) w = hi[a];
-        else w = cmul(hi[a], lo[b]);
+        if (a == 0) w = f.lo[b];
+        else if (b == 0) w = f.hi[a];
+        else w = cmul(f.hi[a], f.lo[b]);
         x[r] = cmul(x[r], w);
     }
+}
+
+// x[r] *= W^(r*e) for r = 1..R-1
+template <int R>
+__device__ __forceinline__ void twiddle_pow(float2 (&x)[R], const float2* __restrict__ tw, int e)
+{
+    TwFac<R> f;
+    twiddle_fac<R>(f, tw, e);
+    twiddle_apply<R>(x, f);
 }
 
 // --------------------------------------------------------------------------
@@ -526,8 +544,11 @@ int fx_fstage_group_size(int prec)
 
 // TWS is the stride of the twiddle table: 1 when tw holds exp(-2 pi i k / 2^LG),
 // 2 when a 2^LG-point sub-transform reads the table of a 2^(LG+1)-point one.
+// fac: when not null, the thread's twiddle factors of this pass (one per
+// butterfly m), loaded before the FFT loop, in place of the table tw.
 template <class Gm, int R, int P, bool TW_LDS, int TWS = 1>
-__device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const float2* __restrict__ tw, int tid)
+__device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const float2* __restrict__ tw, int tid,
+                                         const TwFac<R>* fac = nullptr)
 {
     constexpr int F = Gm::F, NT = Gm::NT, FP = Gm::FP;
     constexpr int T = F / R, PER = T / NT;
@@ -551,6 +572,8 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
             const float2* tk = tw1 + k * (R + 1);
 #pragma unroll
             for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tk[r]);
+        } else if (fac) {
+            twiddle_apply<R>(x[m], fac[m]);
         } else {
             twiddle_pow<R>(x[m], tw, k * (TWS * F / (P * R)));
         }
@@ -627,6 +650,16 @@ __device__ __forceinline__ int dif_freq(int q)
         const int k1 = q % R1, k0 = q / R1;
         return k0 + R0 * k1;
     }
+}
+
+// Channel of a thread's output m*RO + r (RO outputs per last-pass butterfly).
+template <class Gm>
+__device__ __forceinline__ int out_chan(int tid, int m, int r)
+{
+    if constexpr (Gm::KIND == 2)      // cross-lane plan: channel k + 256 (m + 8 up) (+ F/2 for r = 1)
+        return ((tid >> 6) * 32 + (tid & 31)) + 256 * (m + 8 * ((tid >> 5) & 1)) + r * (Gm::F / 2);
+    const int i = tid + m * Gm::NT;
+    return (Gm::KIND == 1 ? dif_freq<Gm>(i) : i) + r * Gm::PL;
 }
 
 // --------------------------------------------------------------------------
@@ -821,6 +854,35 @@ void fx_fstage_kernel(FStageArgs args)
         xl_wk = up ? make_float2(w.y, -w.x) : w;
     }
 
+    // FP8: the inverse scales of this signal's output channels, loaded once
+    // where the register budget allows (three waves per SIMD)
+    constexpr bool SREG = PREC == FX_PREC_F8 && fstage_wpe<LG, V, PREC, CPLX>() == 3;
+    [[maybe_unused]] float sreg[SREG ? NOUT : 1];
+    if constexpr (SREG) {
+#pragma unroll
+        for (int m = 0; m < PERL; m++)
+#pragma unroll
+            for (int r = 0; r < RO; r++)
+                sreg[m * RO + r] = args.scale_inv[(long long)sig * NCH + out_chan<Gm>(threadIdx.x, m, r)];
+    }
+
+    // Twiddle factors whose exponents depend only on the thread, loaded once
+    // where the register budget allows (three waves per SIMD): the third
+    // pass's of a four-pass Stockham plan and the last pass's.
+    constexpr bool TWREG = fstage_wpe<LG, V, PREC, CPLX>() == 3 && !XL && !DIF;
+    constexpr int PER2 = (F / R2) / NT > 0 ? (F / R2) / NT : 1;
+    [[maybe_unused]] TwFac<R2> fac2[TWREG && Gm::NP == 4 ? PER2 : 1];
+    [[maybe_unused]] TwFac<RL> facl[TWREG ? PERL : 1];
+    if constexpr (TWREG) {
+        if constexpr (Gm::NP == 4) {
+#pragma unroll
+            for (int m = 0; m < PER2; m++)
+                twiddle_fac<R2>(fac2[m], args.tw, ((int)threadIdx.x + m * NT) % (R0 * R1) * (F / (R0 * R1 * R2)));
+        }
+#pragma unroll
+        for (int m = 0; m < PERL; m++) twiddle_fac<RL>(facl[m], args.tw, (int)threadIdx.x + m * NT);
+    }
+
 #pragma unroll 1
     for (int gi = 0; gi < G; gi++) {
         // Redefine tid for the compiler on every FFT: the LDS addresses derived
@@ -989,14 +1051,16 @@ void fx_fstage_kernel(FStageArgs args)
                 {
                     const float2 y = cmul(cmul(frs, frb[4 * r]), cadd(av[r], wb));
                     float si = 1.0f;
-                    if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
+                    if constexpr (SREG) si = sreg[r * RO];
+                    else if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
                     outr.put_last(r * RO, y, si);
                     if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
                 }
                 if constexpr (CPLX) {                   // channel c + F/2 = a - W b
                     const float2 y = cmul(cmul(frs, frb[4 * r + 64]), csub(av[r], wb));
                     float si = 1.0f;
-                    if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c + F / 2];
+                    if constexpr (SREG) si = sreg[r * RO + 1];
+                    else if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c + F / 2];
                     outr.put_last(r * RO + 1, y, si);
                     if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c + F / 2] = y;
                 }
@@ -1008,7 +1072,8 @@ void fx_fstage_kernel(FStageArgs args)
             if constexpr (Gm::NP == 4) dif_pass<Gm, R2, F / (R0 * R1)>(buf, args.tw, tid);
         } else {
             mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
-            if constexpr (Gm::NP == 4) mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid);
+            if constexpr (Gm::NP == 4)
+                mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid, TWREG ? fac2 : nullptr);
         }
 
         // ---- last pass: radix RL, the channels (real input: bins < nchan),
@@ -1037,7 +1102,8 @@ void fx_fstage_kernel(FStageArgs args)
                 const float2* rd = rdb + Gm::pad(m * NT);
 #pragma unroll
                 for (int r = 0; r < RL; r++) x[r] = rd[Gm::pad(r * TL)];
-                twiddle_pow<RL>(x, args.tw, i);
+                if constexpr (TWREG) twiddle_apply<RL>(x, facl[m]);
+                else twiddle_pow<RL>(x, args.tw, i);
                 c0 = i;
             }
             dft<RL, !CPLX>(x);
@@ -1049,7 +1115,8 @@ void fx_fstage_kernel(FStageArgs args)
                 else fr = cmul(frs, frb[(m * NT + r * PL) / STRIDE]);
                 const float2 y = cmul(fr, x[r]);
                 float si = 1.0f;
-                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
+                if constexpr (SREG) si = sreg[m * RO + r];
+                else if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
                 outr.put_last(m * RO + r, y, si);
                 if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
             }
@@ -1061,15 +1128,10 @@ void fx_fstage_kernel(FStageArgs args)
     tid = threadIdx.x;
 #pragma unroll
     for (int m = 0; m < PERL; m++) {
-        const int i = tid + m * NT;
-        int c0 = i;
-        if constexpr (DIF) c0 = dif_freq<Gm>(i);
 #pragma unroll
         for (int r = 0; r < RO; r++) {
-            int c = c0 + r * PL;
+            const int c = out_chan<Gm>(tid, m, r);
             const int ou = m * RO + r;
-            // XL: output m*RO + r is channel k + 256 (m + 8 up) (+ F/2 for r = 1)
-            if constexpr (XL) c = ((tid >> 6) * 32 + (tid & 31)) + 256 * (m + 8 * ((tid >> 5) & 1)) + r * (F / 2);
             const long long row = xrow0 + (long long)c * args.kc * p.Spad;
             if constexpr (PREC == FX_PREC_F16) {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
