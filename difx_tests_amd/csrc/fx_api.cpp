@@ -1155,8 +1155,8 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
     const int best_tile = tiles[bt];
     c->xsb = best_tile;
     // subints per X-engine launch: whole steps of k F-stages and one X-engine
-    // launch over their slots; the smallest k within 3% of the best time per
-    // subint
+    // launch over their slots; the smallest k within 1% of the best time per
+    // subint (a larger k only adds buffer memory and latency)
     double per_sub[8];
     int counts[8], nk = 0;
     for (int k = 1; rc == FX_OK && nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) {
@@ -1178,7 +1178,7 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
         double m = per_sub[0];
         for (int i = 1; i < nk; i++) m = per_sub[i] < m ? per_sub[i] : m;
         for (int i = 0; i < nk; i++)
-            if (per_sub[i] <= 1.03 * m) {
+            if (per_sub[i] <= 1.01 * m) {
                 best_k = counts[i];
                 break;
             }

@@ -883,7 +883,9 @@ void fx_fstage_kernel(FStageArgs args)
         for (int m = 0; m < PERL; m++) twiddle_fac<RL>(facl[m], args.tw, (int)threadIdx.x + m * NT);
     }
 
-#pragma unroll 1
+    // two FFTs per loop trip: half the register moves of the output chain and
+    // the prefetch per FFT, at twice the code (four measured slower: I-cache)
+#pragma unroll 2
     for (int gi = 0; gi < G; gi++) {
         // Redefine tid for the compiler on every FFT: the LDS addresses derived
         // from it are then formed where they are used instead of being hoisted
