@@ -189,11 +189,14 @@ __device__ __forceinline__ float2 sincos_rounded(float xf)
 // FP64 phase arguments in the reference's operation order (kernel.c:428-497,
 // 512-555; contraction off), sin/cos rounded to FP32.
 
-// Rotator tables of one (FFT, antenna) from its delay entry, block-strided:
-// sub fine and nstep step fringe rotators (FP64 phases, kernel.c:428-497),
-// stride fine and nfstep step fractional-delay rotators (kernel.c:512-555).
+// Rotator tables of one (FFT, antenna) from its delay entry: sub fine and
+// nstep step fringe rotators (FP64 phases, kernel.c:428-497), stride fine and
+// nfstep step fractional-delay rotators (kernel.c:512-555); this thread's
+// entries first, first + step, ...  F is a power of two, so the reference's
+// divisions by F are the exact products with 1/F.
 __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __restrict__ subchf,
-                             const float* __restrict__ stepchf, int step_q, float2* __restrict__ out)
+                             const float* __restrict__ stepchf, int step_q, float2* __restrict__ out, int first,
+                             int step)
 {
 #pragma clang fp contract(off)
     const int ntab = p.sub + p.nstep + p.stride + p.nfstep;
@@ -204,18 +207,19 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
     const double bfrac = d.b - integerdelay;
     const double lofr = p.lofrac * (double)integerdelay;
     const float fd32 = (float)d.fd;
-    for (int e = threadIdx.x; e < ntab; e += blockDim.x) {
+    const double inv_f = 1.0 / (double)p.F;                     // exact
+    for (int e = first; e < ntab; e += step) {
         float arg;
         bool unit = false;
         if (e < p.sub) {
-            double subxval = ((double)e / (double)p.F) * d.a;
+            double subxval = ((double)e * inv_f) * d.a;
             subxval += bfrac;
             double ph = subxval * p.lo;
             if (p.fracLO) ph += lofr;
             arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
         } else if (e < p.sub + p.nstep) {
             const int k = e - p.sub;
-            const double stepxval = (((double)k * (double)p.stride) / (double)p.F) * d.a;
+            const double stepxval = (((double)k * (double)p.stride) * inv_f) * d.a;
             const double ph = stepxval * p.lo;
             arg = (float)((-2.0 * kPi) * (ph - (double)(int)ph));
             unit = (k == 0);
@@ -235,34 +239,34 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
     }
 }
 
-// One block per (FFT k of the call, antenna j): thread 0 forms the delay
-// entry (global FFT index iteration[k], or it0 + k when iteration is null;
-// buffer origin *origin_dev, or `origin` when that is null), writes it and
-// the FFT slot's validity byte, and the block fills the rotator tables.
-__global__ void fx_delay_rot_kernel(FxPlan p, const double* __restrict__ delays, const double* __restrict__ fileoff,
-                                    const int* __restrict__ iteration, int it0, int numffts, int maxoffset,
-                                    const int* __restrict__ origin_dev, int origin, const float* __restrict__ subchf,
-                                    const float* __restrict__ stepchf, int step_q, FxDelay* __restrict__ dtab,
-                                    float2* __restrict__ rtab, uint8_t* __restrict__ valid_slots)
+// One wave per (FFT k of the call, antenna j), kDelayWaves per block: every
+// lane forms the delay entry (global FFT index iteration[k], or it0 + k when
+// iteration is null; buffer origin *origin_dev, or `origin` when that is
+// null), lane 0 writes it and the FFT slot's validity byte, and the wave fills
+// the rotator tables.  No barrier: the entry is the same in every lane.
+static constexpr int kDelayWaves = 4;
+__global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
+    FxPlan p, const double* __restrict__ delays, const double* __restrict__ fileoff, const int* __restrict__ iteration,
+    int it0, int numffts, long long nfa, int maxoffset, const int* __restrict__ origin_dev, int origin,
+    const float* __restrict__ subchf, const float* __restrict__ stepchf, int step_q, FxDelay* __restrict__ dtab,
+    float2* __restrict__ rtab, uint8_t* __restrict__ valid_slots)
 {
-    __shared__ FxDelay sd;
-    const long long fa = blockIdx.x;
+    const long long fa = (long long)blockIdx.x * kDelayWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (fa >= nfa) return;
     const int k = (int)(fa / p.nant), j = (int)(fa - (long long)k * p.nant);
-    if (threadIdx.x == 0) {
-        FxDelay e;
-        if (k >= numffts) {
-            e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
-        } else {
-            const int g = iteration ? iteration[k] : it0 + k;
-            e = delay_entry(p, delays, fileoff, j, g, origin_dev ? *origin_dev : origin, maxoffset);
-        }
+    FxDelay e;
+    if (k >= numffts) {
+        e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
+    } else {
+        const int g = iteration ? iteration[k] : it0 + k;
+        e = delay_entry(p, delays, fileoff, j, g, origin_dev ? *origin_dev : origin, maxoffset);
+    }
+    if (lane == 0) {
         dtab[fa] = e;
         valid_slots[fa] = (uint8_t)e.valid;
-        sd = e;
     }
-    __syncthreads();
-    const FxDelay d = sd;
-    if (d.valid) rotator_fill(p, d, subchf, stepchf, step_q, rtab + fa * fx_rtab_len(p));
+    if (e.valid) rotator_fill(p, e, subchf, stepchf, step_q, rtab + fa * fx_rtab_len(p), lane, 64);
 }
 
 hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
@@ -274,9 +278,9 @@ hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const do
     if (n <= 0) return hipSuccess;
     if (step_q < 0 || (step_q & (step_q - 1)) != 0 || (p.nstep & (p.nstep - 1)) != 0 || step_q > p.nstep)
         return hipErrorInvalidValue;
-    fx_delay_rot_kernel<<<(unsigned)n, 128, 0, s>>>(p, delays, fileoff, iteration, it0, numffts, maxoffset,
-                                                     origin_dev, origin, subchf, stepchf, step_q, dtab, rtab,
-                                                     valid_slots);
+    fx_delay_rot_kernel<<<(unsigned)((n + kDelayWaves - 1) / kDelayWaves), 64 * kDelayWaves, 0, s>>>(
+        p, delays, fileoff, iteration, it0, numffts, n, maxoffset, origin_dev, origin, subchf, stepchf, step_q, dtab,
+        rtab, valid_slots);
     return hipGetLastError();
 }
 
@@ -854,9 +858,26 @@ void fx_fstage_kernel(FStageArgs args)
         xl_wk = up ? make_float2(w.y, -w.x) : w;
     }
 
+    // XL at G = 4 (FP16; the FFT loop fully unrolled leaves the registers): the
+    // third pass's fifteen twiddles and the last step's eight, formed once per
+    // block (at G = 2, FP32, forming them for two FFTs measured slower)
+    constexpr bool XLH = XL && G == 4 && !CPLX;
+    [[maybe_unused]] float2 xw3[XLH ? 16 : 1], xwl[XLH ? 8 : 1];
+    if constexpr (XLH) {
+        TwFac<16> f3;
+        twiddle_fac<16>(f3, args.tw, ((threadIdx.x >> 6) * 32 + (threadIdx.x & 31)) * (F / (256 * 16)));
+        xw3[0] = make_float2(1.0f, 0.0f);
+#pragma unroll
+        for (int r = 1; r < 16; r++) {
+            const int a = r >> 2, b = r & 3;
+            xw3[r] = a == 0 ? f3.lo[b] : b == 0 ? f3.hi[a] : cmul(f3.hi[a], f3.lo[b]);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) xwl[r] = twc(xl_wk, r);
+    }
     // FP8: the inverse scales of this signal's output channels, loaded once
     // where the register budget allows (three waves per SIMD)
-    constexpr bool SREG = PREC == FX_PREC_F8 && fstage_wpe<LG, V, PREC, CPLX>() == 3;
+    constexpr bool SREG = PREC == FX_PREC_F8 && fstage_wpe<LG, V, PREC, CPLX>() == 3 && !CPLX;
     [[maybe_unused]] float sreg[SREG ? NOUT : 1];
     if constexpr (SREG) {
 #pragma unroll
@@ -869,7 +890,7 @@ void fx_fstage_kernel(FStageArgs args)
     // Twiddle factors whose exponents depend only on the thread, loaded once
     // where the register budget allows (three waves per SIMD): the third
     // pass's of a four-pass Stockham plan and the last pass's.
-    constexpr bool TWREG = fstage_wpe<LG, V, PREC, CPLX>() == 3 && !XL && !DIF;
+    constexpr bool TWREG = fstage_wpe<LG, V, PREC, CPLX>() == 3 && !XL && !DIF && !CPLX;
     constexpr int PER2 = (F / R2) / NT > 0 ? (F / R2) / NT : 1;
     [[maybe_unused]] TwFac<R2> fac2[TWREG && Gm::NP == 4 ? PER2 : 1];
     [[maybe_unused]] TwFac<RL> facl[TWREG ? PERL : 1];
@@ -883,9 +904,12 @@ void fx_fstage_kernel(FStageArgs args)
         for (int m = 0; m < PERL; m++) twiddle_fac<RL>(facl[m], args.tw, (int)threadIdx.x + m * NT);
     }
 
-    // two FFTs per loop trip: half the register moves of the output chain and
-    // the prefetch per FFT, at twice the code (four measured slower: I-cache)
-#pragma unroll 2
+    // The FFT loop is unrolled (G <= 4) or, for FP8 (G = 8, where eight copies
+    // measured slower: I-cache), runs two FFTs per trip where the register
+    // budget allows (three waves per SIMD): no register moves of the output
+    // chain and the prefetch between the FFTs of one trip.
+    constexpr int UNR = G <= 4 ? G : (fstage_wpe<LG, V, PREC, CPLX>() == 3 && !CPLX) ? 2 : 1;
+#pragma unroll UNR
     for (int gi = 0; gi < G; gi++) {
         // Redefine tid for the compiler on every FFT: the LDS addresses derived
         // from it are then formed where they are used instead of being hoisted
@@ -1028,7 +1052,12 @@ void fx_fstage_kernel(FStageArgs args)
             const float2* rd = buf + Gm::pad(k + 256 * up);
 #pragma unroll
             for (int r = 0; r < 16; r++) x[r] = rd[Gm::pad(r * T2)];
-            twiddle_pow<16>(x, args.tw, k * (F / (256 * 16)));
+            if constexpr (XLH) {
+#pragma unroll
+                for (int r = 1; r < 16; r++) x[r] = cmul(x[r], xw3[r]);
+            } else {
+                twiddle_pow<16>(x, args.tw, k * (F / (256 * 16)));
+            }
             dft<16>(x);
             // x[r] is A[k + 256 r] (lanes 0-31) or B[k + 256 r] (lanes 32-63):
             // trade halves so each lane holds A and B of rho = r + 8*up
@@ -1048,7 +1077,7 @@ void fx_fstage_kernel(FStageArgs args)
             const float2* frb = frstep + (k >> 6) + 32 * up;
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                const float2 wb = cmul(bv[r], twc(xl_wk, r));
+                const float2 wb = cmul(bv[r], XLH ? xwl[r] : twc(xl_wk, r));
                 const int c = k + 256 * (r + 8 * up);
                 {
                     const float2 y = cmul(cmul(frs, frb[4 * r]), cadd(av[r], wb));
