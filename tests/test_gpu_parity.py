@@ -314,6 +314,34 @@ def test_radix_plans_agree_in_low_precision(nchan, prec):
         assert _rel(o[0][live], out[0][0][live]).max() <= tol
 
 
+@pytest.mark.parametrize("prec", ["f16", "f8"])
+@pytest.mark.parametrize("plan", [0, 1, 2])
+@pytest.mark.parametrize("nchan,nf", [(512, 32), (1024, 32), (2048, 24), (4096, 16), (8192, 16)])
+def test_vis_lowprec_every_plan_vs_fp64_oracle(nchan, nf, plan, prec):
+    # every radix plan of the FP16 / FP8 F-stage against the FP64 oracle: the
+    # per-FFT state kept in registers across the FFT loop (twiddle factors,
+    # FP8 scales, the cross-lane plan's twiddles) differs by plan and precision.
+    # FP8's error in the autos falls as 1/sqrt(FFTs): at least 16 FFTs keep
+    # it under the 2% bound (8 FFTs at 8192 channels measured 2.2%)
+    nant = 5
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 500 + nchan + 7 * plan, data="correlated")
+    with _engine(p, nant, nf, prec=prec, data=data) as e:
+        if plan >= e.nplans():
+            pytest.skip(f"{e.nplans()} radix plans at nchan={nchan}")
+        e.set_plan(plan)
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f64(p, data, delays, fileoff, it)
+    assert np.array_equal(cc, ref["count"]) and np.array_equal(ac, ref["autocount"])
+    live = cc > 0
+    tol, tola = (1e-3, 1e-3) if prec == "f16" else (0.1, 0.02)
+    assert _rel(cross[live], ref["vis"][live]).max() <= tol
+    assert _rel(autos[:, [0, 3]], ref["autos"][:, [0, 3]]).max() <= tola
+
+
 def test_autotune_and_cache(tmp_path, monkeypatch):
     nchan, nant, nf = 2048, 6, 12
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 71)
