@@ -136,10 +136,26 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const int SJ = SI + pr;
     const bool diag = SI == SJ;
 
-    // blocks of this wave
+    // blocks of this wave.  SB = 4: a 2x2 sub-tile per wave (rows
+    // 2*(wave/2) + i, columns 2*(wave%2) + j in slot t = 2i + j), so each K
+    // step reads two A and two B fragments from the LDS for four block
+    // products; on a diagonal pair the sub-tile below the diagonal is idle and
+    // so are the blocks below it.  Otherwise blocks are dealt round-robin.
+    constexpr bool SUBT = SB == 4;
+    static_assert(!SUBT || (WAVES == 4 && NBW == 4), "2x2 sub-tiles of a 4x4 pair");
     int bI[NBW], bJ[NBW];
     unsigned okm = 0;                                     // bit t: slot t holds a block
-    {
+    if constexpr (SUBT) {
+#pragma unroll
+        for (int t = 0; t < NBW; t++) {
+            const int bi = 2 * (wave >> 1) + (t >> 1), bj = 2 * (wave & 1) + (t & 1);
+            const int I = SI * SB + bi, J = SJ * SB + bj;
+            const bool ok = (!diag || bi <= bj) && I < nblk && J < nblk;
+            bI[t] = bi;
+            bJ[t] = bj;
+            okm |= (unsigned)ok << t;
+        }
+    } else {
         const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
 #pragma unroll
         for (int t = 0; t < NBW; t++) {
@@ -227,7 +243,29 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
         for (int sub = 0; sub < NSUB; sub++) {
         if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
-        if constexpr (PREC == FX_PREC_F16) {
+        if constexpr (PREC == FX_PREC_F16 && SUBT) {
+            const int r0 = 2 * (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
+#pragma unroll
+            for (int ks = 0; ks < KC / 2; ks++) {
+                half8 hb[2];
+#pragma unroll
+                for (int j = 0; j < 2; j++) hb[j] = as_half8(lds[stg][sub][pb][2 * ks + kg][c0 + j * FX_BLK]);
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    if (((okm >> (2 * i)) & 3u) == 0) continue;
+                    const uint4 A = lds[stg][sub][0][2 * ks + kg][r0 + i * FX_BLK];
+                    const half8 ha = as_half8(A), hc = as_half8(conj_swap_f16(A));
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        const int t = 2 * i + j;
+                        if ((okm >> t) & 1u) {
+                            accr[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ha, hb[j], accr[t], 0, 0, 0);
+                            acci[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hc, hb[j], acci[t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        } else if constexpr (PREC == FX_PREC_F16) {
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
 #pragma unroll
@@ -258,6 +296,42 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
                         for (int f = 0; f < 2; f++) {
                             accr[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[f], bv[f], accr[t], 0, 0, 0);
                             acci[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[f], bv[f], acci[t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        } else if constexpr (SUBT) {
+            // FP8 e4m3, 2x2 sub-tile: K step = 4 groups (32 FFTs); lane kg
+            // supplies groups 2kg, 2kg+1
+            const int r0 = 2 * (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ks++) {
+                i32x8 bv[2];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const uint4 b0 = lds[stg][sub][pb][4 * ks + 2 * kg][c0 + j * FX_BLK];
+                    const uint4 b1 = lds[stg][sub][pb][4 * ks + 2 * kg + 1][c0 + j * FX_BLK];
+                    bv[j] = (i32x8){(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
+                                    (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+                }
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    if (((okm >> (2 * i)) & 3u) == 0) continue;
+                    const uint4 a0 = lds[stg][sub][0][4 * ks + 2 * kg][r0 + i * FX_BLK];
+                    const uint4 a1 = lds[stg][sub][0][4 * ks + 2 * kg + 1][r0 + i * FX_BLK];
+                    const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                                      (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                    i32x8 cv;
+#pragma unroll
+                    for (int q = 0; q < 8; q++) cv[q] = (int)conj_swap_f8w((uint32_t)av[q]);
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        const int t = 2 * i + j;
+                        if ((okm >> t) & 1u) {
+                            accr[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv[j], accr[t], 0, 0, 0,
+                                                                                     127, 0, 127);
+                            acci[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, bv[j], acci[t], 0, 0, 0,
+                                                                                     127, 0, 127);
                         }
                     }
                 }
