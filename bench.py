@@ -35,6 +35,9 @@ from difx_tests_amd import FxEngine, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"f16": 2500.0, "f8": 5000.0, "f32": 157.3, "f32v": 157.3}
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4
+# cycles at 2.4 GHz (no packed FP32)
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
 OUT_BYTES_PER_SAMPLE = {"f32": 4.0, "f16": 2.0, "f8": 1.0, "f32v": 4.0}   # nchan complex bins per F real samples
 
 
@@ -244,6 +247,20 @@ def measured_traffic(workload, prec, xbatch, kernel):
     return None, None
 
 
+def measured_valu(workload, prec, kernel):
+    """VALU instructions (wave64) per launch of `kernel` from the committed
+    rocprofv3 SQ_INSTS_VALU summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_{workload}_{prec}.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "valu_insts_per_launch" and parts[1] == kernel:
+                return int(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(w, data, delays, fileoff, budget_s):
     """Single-thread FP32 restatement (oracle) on a bounded FFT sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -314,6 +331,15 @@ def main():
                 "algorithmic_flops": int(x_flops), "avg_launch_ms": round(x_ms, 4)}
     if src:
         roof["traffic_source"] = src
+    # the F-stage's real limiter: VALU issue (non-packed FP32), its profiled
+    # instruction count over the live launch time against the issue peak
+    valu_roof = None
+    ninst, vsrc = measured_valu(w.name, prec, "fx_fstage_kernel")
+    if ninst:
+        g = ninst / (f_ms * 1e-3) / 1e9
+        valu_roof = {"kernel": "fx_fstage_kernel", "bound": "valu", "achieved": round(g, 1),
+                     "peak": VALU_PEAK_GINST, "unit": "G wave64-instructions/s",
+                     "frac": round(g / VALU_PEAK_GINST, 4), "insts_per_launch": ninst, "source": vsrc}
 
     out = {
         "metric": "input Gsamples/s/GPU + X-engine TFLOPS; FP16-MFMA vs FP32 at <=1e-3 rel err",
@@ -337,6 +363,7 @@ def main():
         "realtime_factor": round((xbatch * w.subint_s) / sec, 2),
         "tuning": tuned,
         "roofline": roof,
+        "valu_roofline": valu_roof,
     }
     if not a.no_fp32_compare and prec not in ("f32", "f32v"):
         # the plain FP32 GPU pipeline (vector-ALU X-engine) and the FP32 MFMA one
