@@ -1137,7 +1137,10 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
         if (fms[v] < fms[best_plan]) best_plan = v;
     double best_f = fms[best_plan];
     c->fplan = best_plan;
-    // X-engine tiles over one subint
+    // X-engine tiles over up to 8 subints of slots (the K depth of a batched
+    // launch: a tile's K pipeline is not in its steady state over one subint)
+    int kt = 1;
+    while (kt < 8 && (long long)2 * kt * f.npad <= c->max_ffts) kt *= 2;
     const int tiles[3] = {1, 2, 4};
     double xms[3] = {1e30, 1e30, 1e30};
     for (int round = 0; rc == FX_OK && round < kRounds; round++)
@@ -1145,8 +1148,8 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
             if (tiles[t] > 1 && c->p.nblk < tiles[t] - 1) continue;   // tile far larger than the signal count
             c->xsb = tiles[t];
             double ms = 0.0;
-            rc = tune_time(c, reps, &ms, [&] { return xstage_enqueue(c, 0, f.npad, c->stream); });
-            if (rc == FX_OK) xms[t] = std::min(xms[t], ms);
+            rc = tune_time(c, reps, &ms, [&] { return xstage_enqueue(c, 0, kt * f.npad, c->stream); });
+            if (rc == FX_OK) xms[t] = std::min(xms[t], ms / kt);
         }
     int bt = 0;
     for (int t = 1; t < 3; t++)
