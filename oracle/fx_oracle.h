@@ -21,7 +21,7 @@
  *   - FP64 oracle: the same discrete decisions (integer offsets, validity) and
  *     the same rotator factorisation, with every floating operation in FP64
  *     and no narrowing.  This is the truth against which the reduced-precision
- *     (FP16 / FP8) X-engines are judged.
+ *     (FP16 / FP8) X-engines are measured.
  *
  * Generalised channel counts: the reference accepts only square nchan
  * (kernel.c:31-35).  For any power-of-two nchan the oracle uses
