@@ -482,52 +482,59 @@ template <int PREC, int G, int NOUT> struct OutRegs;
 // FFT loop can stay rolled while every register index is static.
 template <int G, int NOUT> struct OutRegs<FX_PREC_F32, G, NOUT> {
     float2 v[G][NOUT];
-    __device__ __forceinline__ void shift() {
+    __device__ __forceinline__ void shift(int) {
 #pragma unroll
         for (int g = 0; g + 1 < G; g++)
 #pragma unroll
             for (int u = 0; u < NOUT; u++) v[g][u] = v[g + 1][u];
     }
-    __device__ __forceinline__ void zero_last() {
+    __device__ __forceinline__ void zero_last(int) {
 #pragma unroll
         for (int u = 0; u < NOUT; u++) v[G - 1][u] = make_float2(0.0f, 0.0f);
     }
-    __device__ __forceinline__ void put_last(int u, float2 x, float) { v[G - 1][u] = x; }
+    __device__ __forceinline__ void put_last(int u, float2 x, float, int) { v[G - 1][u] = x; }
 };
 template <int G, int NOUT> struct OutRegs<FX_PREC_F16, G, NOUT> {
     uint32_t v[G][NOUT];
-    __device__ __forceinline__ void shift() {
+    __device__ __forceinline__ void shift(int) {
 #pragma unroll
         for (int g = 0; g + 1 < G; g++)
 #pragma unroll
             for (int u = 0; u < NOUT; u++) v[g][u] = v[g + 1][u];
     }
-    __device__ __forceinline__ void zero_last() {
+    __device__ __forceinline__ void zero_last(int) {
 #pragma unroll
         for (int u = 0; u < NOUT; u++) v[G - 1][u] = 0u;
     }
-    __device__ __forceinline__ void put_last(int u, float2 x, float) {
+    __device__ __forceinline__ void put_last(int u, float2 x, float, int) {
         const __half2 h = __floats2half2_rn(x.x, x.y);
         v[G - 1][u] = *reinterpret_cast<const uint32_t*>(&h);
     }
 };
 // FP8: a word holds two FFTs (re,im e4m3 each) as 16-bit halves, slot order
-// (low, high) = (earlier, later); the chain shifts by half-words.
+// (low, high) = (earlier, later).  FFT gi of the group enters half gi & 1 of
+// the last word, and before each even FFT the words move down by one: one
+// word move per output and FFT pair instead of a half-word shift per FFT.
 template <int G, int NOUT> struct OutRegs<FX_PREC_F8, G, NOUT> {
     uint32_t v[G / 2][NOUT];
-    __device__ __forceinline__ void shift() {
+    __device__ __forceinline__ void shift(int gi) {
+        if (gi & 1) return;
 #pragma unroll
-        for (int u = 0; u < NOUT; u++) {
+        for (int u = 0; u < NOUT; u++)
 #pragma unroll
-            for (int w = 0; w + 1 < G / 2; w++) v[w][u] = (v[w][u] >> 16) | (v[w + 1][u] << 16);
-            v[G / 2 - 1][u] >>= 16;
-        }
+            for (int w = 0; w + 1 < G / 2; w++) v[w][u] = v[w + 1][u];
     }
-    __device__ __forceinline__ void zero_last() {}
-    __device__ __forceinline__ void put_last(int u, float2 x, float si) {
+    __device__ __forceinline__ void zero_last(int gi) {
+#pragma unroll
+        for (int u = 0; u < NOUT; u++) v[G / 2 - 1][u] = (gi & 1) ? (v[G / 2 - 1][u] & 0xffffu) : 0u;
+    }
+    __device__ __forceinline__ void put_last(int u, float2 x, float si, int gi) {
         const float a = fminf(fmaxf(x.x * si, -448.0f), 448.0f);
         const float b = fminf(fmaxf(x.y * si, -448.0f), 448.0f);
-        v[G / 2 - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, (int)v[G / 2 - 1][u], true);
+        if (gi & 1)
+            v[G / 2 - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, (int)v[G / 2 - 1][u], true);
+        else
+            v[G / 2 - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
     }
 };
 
@@ -916,11 +923,11 @@ void fx_fstage_kernel(FStageArgs args)
         // out of the loop and held in registers across all passes.
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid >= 0 && tid < NT);
-        outr.shift();
+        outr.shift(gi);
         const int fft = grp * G + gi;             // FFT of this launch
         __syncthreads();                          // previous FFT done with LDS
         if (!cur.valid) {                         // uniform across the block
-            outr.zero_last();
+            outr.zero_last(gi);
             if (TAP) {
 #pragma unroll
                 for (int m = 0; m < PERL; m++)
@@ -1093,7 +1100,7 @@ void fx_fstage_kernel(FStageArgs args)
                     float si = 1.0f;
                     if constexpr (SREG) si = sreg[r * RO];
                     else if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
-                    outr.put_last(r * RO, y, si);
+                    outr.put_last(r * RO, y, si, gi);
                     if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
                 }
                 if constexpr (CPLX) {                   // channel c + F/2 = a - W b
@@ -1101,7 +1108,7 @@ void fx_fstage_kernel(FStageArgs args)
                     float si = 1.0f;
                     if constexpr (SREG) si = sreg[r * RO + 1];
                     else if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c + F / 2];
-                    outr.put_last(r * RO + 1, y, si);
+                    outr.put_last(r * RO + 1, y, si, gi);
                     if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c + F / 2] = y;
                 }
             }
@@ -1157,7 +1164,7 @@ void fx_fstage_kernel(FStageArgs args)
                 float si = 1.0f;
                 if constexpr (SREG) si = sreg[m * RO + r];
                 else if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
-                outr.put_last(m * RO + r, y, si);
+                outr.put_last(m * RO + r, y, si, gi);
                 if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
             }
         }
