@@ -1154,17 +1154,13 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
     int bt = 0;
     for (int t = 1; t < 3; t++)
         if (xms[t] < xms[bt]) bt = t;
-    double best_x = xms[bt];
-    const int best_tile = tiles[bt];
+    int best_tile = tiles[bt];
     c->xsb = best_tile;
     // subints per X-engine launch: whole steps of k F-stages and one X-engine
     // launch over their slots; the smallest k within 1% of the best time per
     // subint (a larger k only adds buffer memory and latency)
-    double per_sub[8];
-    int counts[8], nk = 0;
-    for (int k = 1; rc == FX_OK && nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) {
-        double ms = 0.0;
-        rc = tune_time(c, reps, &ms, [&] {
+    auto step_ms = [&](int k, double* ms) {
+        return tune_time(c, reps, ms, [&] {
             for (int s = 0; s < k; s++) {
                 FStageCall fs = f;
                 fs.fft0 = s * f.npad;
@@ -1173,6 +1169,12 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
             }
             return xstage_enqueue(c, 0, k * f.npad, c->stream);
         });
+    };
+    double per_sub[8];
+    int counts[8], nk = 0;
+    for (int k = 1; rc == FX_OK && nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) {
+        double ms = 0.0;
+        rc = step_ms(k, &ms);
         counts[nk] = k;
         per_sub[nk++] = ms / k;
     }
@@ -1187,6 +1189,26 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
             }
     }
     c->xbatch = best_k;
+    // the X tile again, in whole steps of best_k subints: a tile also changes
+    // the F-stages' speed (what the X-engine leaves in the caches, the power
+    // it draws), which timing the X-engine alone does not show
+    {
+        double sms[3] = {1e30, 1e30, 1e30};
+        for (int round = 0; rc == FX_OK && round < kRounds; round++)
+            for (int t = 0; rc == FX_OK && t < 3; t++) {
+                if (xms[t] > 1.25 * xms[bt]) continue;          // far slower alone: not a candidate
+                c->xsb = tiles[t];
+                double ms = 0.0;
+                rc = step_ms(best_k, &ms);
+                if (rc == FX_OK) sms[t] = std::min(sms[t], ms);
+            }
+        int bs = bt;
+        for (int t = 0; t < 3; t++)
+            if (sms[t] < sms[bs]) bs = t;
+        best_tile = tiles[bs];
+        c->xsb = best_tile;
+    }
+    const double best_x = xms[best_tile == 1 ? 0 : best_tile == 2 ? 1 : 2];
     c->timing = saved_timing;
     c->tap = saved_tap;
     if (rc) return rc;

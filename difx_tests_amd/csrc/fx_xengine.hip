@@ -540,7 +540,11 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-int xengine_chunk_groups(int prec) { return prec == FX_PREC_F8 ? XChunk<FX_PREC_F8>::KC : XChunk<FX_PREC_F16>::KC; }
+int xengine_chunk_groups(int prec)
+{
+    return prec == FX_PREC_F8 ? XChunk<FX_PREC_F8>::KC : prec == FX_PREC_F16 ? XChunk<FX_PREC_F16>::KC
+                                                                             : XChunk<FX_PREC_F32>::KC;
+}
 
 
 hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s)
