@@ -141,11 +141,23 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     // step reads two A and two B fragments from the LDS for four block
     // products; on a diagonal pair the sub-tile below the diagonal is idle and
     // so are the blocks below it.  Otherwise blocks are dealt round-robin.
+    // SB = 1 (KSPLIT): the pair is one block; every wave accumulates it over
+    // a quarter of the K steps (K step k to wave k % WAVES) and the four
+    // partial sums are added through the LDS at the end, instead of one wave
+    // computing the block while three idle.
     constexpr bool SUBT = SB == 4;
+    constexpr bool KSPLIT = SB == 1;
     static_assert(!SUBT || (WAVES == 4 && NBW == 4), "2x2 sub-tiles of a 4x4 pair");
     int bI[NBW], bJ[NBW];
     unsigned okm = 0;                                     // bit t: slot t holds a block
-    if constexpr (SUBT) {
+    if constexpr (KSPLIT) {
+#pragma unroll
+        for (int t = 0; t < NBW; t++) {
+            bI[t] = 0;
+            bJ[t] = 0;
+        }
+        okm = (SI < nblk && SJ < nblk) ? 1u : 0u;
+    } else if constexpr (SUBT) {
 #pragma unroll
         for (int t = 0; t < NBW; t++) {
             const int bi = 2 * (wave >> 1) + (t >> 1), bj = 2 * (wave & 1) + (t & 1);
@@ -237,6 +249,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         sstore(0);
     }
     __syncthreads();
+    // KSPLIT: K step k of the launch belongs to wave k % WAVES
+    auto mine = [&](int k) { return !KSPLIT || (k & (WAVES - 1)) == wave; };
     for (int sp = 0; sp < nstep; sp++) {
         const int stg = sp & 1;
         if (sp + 1 < nstep) gload(sp + 1);
@@ -268,6 +282,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         } else if constexpr (PREC == FX_PREC_F16) {
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
+                if (!mine((sp * NSUB + sub) * (KC / 2) + ks)) continue;
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
@@ -284,6 +299,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             // rows (re0, re1, im0, im1); K step = 1 FFT, lanes 0-31 re, 32-63 im
 #pragma unroll
             for (int g = 0; g < KC; g++) {
+                if (!mine((sp * NSUB + sub) * KC + g)) continue;
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
@@ -340,6 +356,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             // FP8 e4m3: K step = 4 groups (32 FFTs); lane kg supplies groups 2kg, 2kg+1
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ks++) {
+                if (!mine((sp * NSUB + sub) * (KC / 4) + ks)) continue;
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
@@ -365,6 +382,33 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         __syncthreads();
     }
 
+    if constexpr (KSPLIT) {
+        // wave 0 adds the other waves' partial sums, one wave at a time
+        // through the staging LDS (free after the loop's last barrier; it
+        // holds at least one wave's 2 x 32 x 32 floats); the order is fixed,
+        // so the result is the same on every run
+        static_assert(sizeof(lds) >= 2 * 16 * 64 * sizeof(float), "staging LDS holds one wave's partial sums");
+        float* red = reinterpret_cast<float*>(&lds[0][0][0][0][0]);
+        for (int w = 1; w < WAVES; w++) {
+            if (wave == w) {
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    red[v * 64 + lane] = accr[0][v];
+                    red[(16 + v) * 64 + lane] = acci[0][v];
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    accr[0][v] += red[v * 64 + lane];
+                    acci[0][v] += red[(16 + v) * 64 + lane];
+                }
+            }
+            __syncthreads();
+        }
+        if (wave != 0) okm = 0;
+    }
     const float* sc = (PREC == FX_PREC_F8) ? a.scale + (long long)c * Spad : nullptr;
 #pragma unroll
     for (int t = 0; t < NBW; t++) {
