@@ -342,6 +342,30 @@ def test_vis_lowprec_every_plan_vs_fp64_oracle(nchan, nf, plan, prec):
     assert _rel(autos[:, [0, 3]], ref["autos"][:, [0, 3]]).max() <= tola
 
 
+@pytest.mark.parametrize("prec", ["f16", "f8"])
+@pytest.mark.parametrize("nchan", [1024, 4096])
+def test_vis_lowprec_invalid_ffts_vs_fp64_oracle(nchan, prec):
+    # net delays that make whole FFTs of an antenna invalid (the first, the
+    # last, clamped edges): the F-stage zeroes those slots of its register
+    # chain (FP8: one half of a word, FFTs in pairs) and the counts skip them
+    nant, nf = 5, 32
+    p, data, delays, fileoff, it = _case(nchan, nant, nf, 600 + nchan, spread=2.4, data="correlated")
+    with _engine(p, nant, nf, prec=prec, data=data) as e:
+        e.set_model(delays, fileoff)
+        e.upload(data)
+        e.reset()
+        e.process(it)
+        cross, cc, autos, ac = e.read_vis()
+    ref = orc.process_f64(p, data, delays, fileoff, it)
+    assert 0 < ref["valid"].sum() < nf * nant          # some FFTs of some antennas are invalid
+    assert np.array_equal(cc, ref["count"]) and np.array_equal(ac, ref["autocount"])
+    live = cc > 0
+    tol, tola = (1e-3, 1e-3) if prec == "f16" else (0.1, 0.02)
+    assert _rel(cross[live], ref["vis"][live]).max() <= tol
+    alive = ac > 0
+    assert _rel(autos[alive][:, [0, 3]], ref["autos"][alive][:, [0, 3]]).max() <= tola
+
+
 def test_autotune_and_cache(tmp_path, monkeypatch):
     nchan, nant, nf = 2048, 6, 12
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 71)
