@@ -106,19 +106,19 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // chunks: the global loads of step k+1 are in flight while the MFMAs of step
 // k run.  NSUB > 1 serves grids too small to fill the GPU (few channels and
 // blocks), whose workgroups would otherwise wait one memory latency per chunk.
-template <int PREC, int SB, int NSUB>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2))) void fx_xengine_kernel(XArgs a)
+template <int PREC, int SB, int NSUB, int NW = WAVES>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
     constexpr int PW = SB * FX_BLK;                      // signals per panel
     constexpr int NELEM = 2 * PW * KC;                   // 16-byte rows per chunk (both panels)
-    constexpr int NLD = (NSUB * NELEM + 64 * WAVES - 1) / (64 * WAVES);
+    constexpr int NLD = (NSUB * NELEM + 64 * NW - 1) / (64 * NW);
     __shared__ __align__(16) uint4 lds[2][NSUB][2][KC][PW];   // [stage][chunk][panel][group][signal]
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
-    // the launch has 64*WAVES threads: the panel of a load slot is then a
+    // the launch has 64*NW threads: the panel of a load slot is then a
     // compile-time function of the slot wherever whole waves share it
-    __builtin_assume(tid >= 0 && tid < 64 * WAVES);
+    __builtin_assume(tid >= 0 && tid < 64 * NW);
     const int Spad = a.p.Spad, nblk = a.p.nblk;
     const int npan = (nblk + SB - 1) / SB;
     const int npairs = npan * (npan + 1) / 2;
@@ -136,18 +136,22 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const int SJ = SI + pr;
     const bool diag = SI == SJ;
 
-    // blocks of this wave.  SB = 4: a 2x2 sub-tile per wave (rows
-    // 2*(wave/2) + i, columns 2*(wave%2) + j in slot t = 2i + j), so each K
-    // step reads two A and two B fragments from the LDS for four block
-    // products; on a diagonal pair the sub-tile below the diagonal is idle and
-    // so are the blocks below it.  Otherwise blocks are dealt round-robin.
-    // SB = 1 (KSPLIT): the pair is one block; every wave accumulates it over
-    // a quarter of the K steps (K step k to wave k % WAVES) and the four
-    // partial sums are added through the LDS at the end, instead of one wave
-    // computing the block while three idle.
-    constexpr bool SUBT = SB == 4;
+    // blocks of this wave.  SB = 4 with eight waves (FP16, FP8; SUBT8): a 1x2
+    // sub-tile per wave (row wave/2, columns 2*(wave%2) + j in slot j), so a
+    // K step reads one A and two B fragments for two block products and
+    // sixteen waves per CU hide the loads (measured against four waves with
+    // 2x2 sub-tiles: the 256-antenna FP8 X-engine 2.95 -> 2.67 ms per 16
+    // subints).  SB = 4 with four waves (FP32; SUBT): a 2x2 sub-tile per wave
+    // (rows 2*(wave/2) + i, columns 2*(wave%2) + j in slot 2i + j).  On a
+    // diagonal pair the blocks below the diagonal are idle.  SB = 2: blocks
+    // dealt round-robin.  SB = 1 (KSPLIT): the pair is one block; every wave
+    // accumulates it over a quarter of the K steps (K step k to wave k % NW)
+    // and the four partial sums are added through the LDS at the end, instead
+    // of one wave computing the block while three idle.
+    constexpr bool SUBT = SB == 4 && NW == 4;
+    constexpr bool SUBT8 = SB == 4 && NW == 8;
     constexpr bool KSPLIT = SB == 1;
-    static_assert(!SUBT || (WAVES == 4 && NBW == 4), "2x2 sub-tiles of a 4x4 pair");
+    static_assert(!SUBT || (NW == 4 && NBW == 4), "2x2 sub-tiles of a 4x4 pair");
     int bI[NBW], bJ[NBW];
     unsigned okm = 0;                                     // bit t: slot t holds a block
     if constexpr (KSPLIT) {
@@ -157,6 +161,16 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
             bJ[t] = 0;
         }
         okm = (SI < nblk && SJ < nblk) ? 1u : 0u;
+    } else if constexpr (SUBT8) {
+#pragma unroll
+        for (int t = 0; t < NBW; t++) {
+            const int bi = wave >> 1, bj = 2 * (wave & 1) + (t & 1);
+            const int I = SI * SB + bi, J = SJ * SB + bj;
+            const bool ok = t < 2 && (!diag || bi <= bj) && I < nblk && J < nblk;
+            bI[t] = bi;
+            bJ[t] = bj;
+            okm |= (unsigned)ok << t;
+        }
     } else if constexpr (SUBT) {
 #pragma unroll
         for (int t = 0; t < NBW; t++) {
@@ -171,7 +185,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
 #pragma unroll
         for (int t = 0; t < NBW; t++) {
-            const int q = wave + t * WAVES;
+            const int q = wave + t * NW;
             int bi = 0, bj = 0;
             if (diag) {
                 int r = q;
@@ -203,14 +217,14 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     // blocks beyond nblk, which okm excludes), and with one chunk per step a
     // diagonal workgroup skips its second panel (a workgroup- or wave-uniform
     // branch; that LDS is not read).
-    static_assert(NELEM % (64 * WAVES) == 0, "whole loads per thread");
+    static_assert(NELEM % (64 * NW) == 0, "whole loads per thread");
     uint4 st[NLD];
 #pragma unroll
     for (int l = 0; l < NLD; l++) st[l] = make_uint4(0, 0, 0, 0);
     auto gload = [&](int sp) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
-            const int e = tid + l * 64 * WAVES;
+            const int e = tid + l * 64 * NW;
             const int sub = e / NELEM, e2 = e - sub * NELEM;
             const int pan = e2 / (KC * PW), rem = e2 - pan * (KC * PW);
             const int g = rem / PW, sgl = rem - g * PW;
@@ -227,7 +241,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     auto sstore = [&](int stage) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
-            const int e = tid + l * 64 * WAVES;
+            const int e = tid + l * 64 * NW;
             const int sub = e / NELEM, e2 = e - sub * NELEM;
             const int pan = e2 / (KC * PW), rem = e2 - pan * (KC * PW);
             const int g = rem / PW, sgl = rem - g * PW;
@@ -249,33 +263,27 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         sstore(0);
     }
     __syncthreads();
-    // KSPLIT: K step k of the launch belongs to wave k % WAVES
-    auto mine = [&](int k) { return !KSPLIT || (k & (WAVES - 1)) == wave; };
+    // KSPLIT: K step k of the launch belongs to wave k % NW
+    auto mine = [&](int k) { return !KSPLIT || (k & (NW - 1)) == wave; };
     for (int sp = 0; sp < nstep; sp++) {
         const int stg = sp & 1;
         if (sp + 1 < nstep) gload(sp + 1);
 #pragma unroll
         for (int sub = 0; sub < NSUB; sub++) {
         if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
-        if constexpr (PREC == FX_PREC_F16 && SUBT) {
-            const int r0 = 2 * (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
+        if constexpr (PREC == FX_PREC_F16 && SUBT8) {
+            const int r0 = (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
-                half8 hb[2];
+                if (!(okm & 3u)) break;
+                const uint4 A = lds[stg][sub][0][2 * ks + kg][r0];
+                const half8 ha = as_half8(A), hc = as_half8(conj_swap_f16(A));
 #pragma unroll
-                for (int j = 0; j < 2; j++) hb[j] = as_half8(lds[stg][sub][pb][2 * ks + kg][c0 + j * FX_BLK]);
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    if (((okm >> (2 * i)) & 3u) == 0) continue;
-                    const uint4 A = lds[stg][sub][0][2 * ks + kg][r0 + i * FX_BLK];
-                    const half8 ha = as_half8(A), hc = as_half8(conj_swap_f16(A));
-#pragma unroll
-                    for (int j = 0; j < 2; j++) {
-                        const int t = 2 * i + j;
-                        if ((okm >> t) & 1u) {
-                            accr[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ha, hb[j], accr[t], 0, 0, 0);
-                            acci[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hc, hb[j], acci[t], 0, 0, 0);
-                        }
+                for (int j = 0; j < 2; j++) {
+                    if ((okm >> j) & 1u) {
+                        const half8 hb = as_half8(lds[stg][sub][pb][2 * ks + kg][c0 + j * FX_BLK]);
+                        accr[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ha, hb, accr[j], 0, 0, 0);
+                        acci[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hc, hb, acci[j], 0, 0, 0);
                     }
                 }
             }
@@ -316,39 +324,27 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
                     }
                 }
             }
-        } else if constexpr (SUBT) {
-            // FP8 e4m3, 2x2 sub-tile: K step = 4 groups (32 FFTs); lane kg
-            // supplies groups 2kg, 2kg+1
-            const int r0 = 2 * (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
+        } else if constexpr (SUBT8 && PREC == FX_PREC_F8) {
+            const int r0 = (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ks++) {
-                i32x8 bv[2];
+                if (!(okm & 3u)) break;
+                const uint4 a0 = lds[stg][sub][0][4 * ks + 2 * kg][r0];
+                const uint4 a1 = lds[stg][sub][0][4 * ks + 2 * kg + 1][r0];
+                const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                                  (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                i32x8 cv;
+#pragma unroll
+                for (int q = 0; q < 8; q++) cv[q] = (int)conj_swap_f8w((uint32_t)av[q]);
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
-                    const uint4 b0 = lds[stg][sub][pb][4 * ks + 2 * kg][c0 + j * FX_BLK];
-                    const uint4 b1 = lds[stg][sub][pb][4 * ks + 2 * kg + 1][c0 + j * FX_BLK];
-                    bv[j] = (i32x8){(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
-                                    (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-                }
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    if (((okm >> (2 * i)) & 3u) == 0) continue;
-                    const uint4 a0 = lds[stg][sub][0][4 * ks + 2 * kg][r0 + i * FX_BLK];
-                    const uint4 a1 = lds[stg][sub][0][4 * ks + 2 * kg + 1][r0 + i * FX_BLK];
-                    const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
-                                      (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-                    i32x8 cv;
-#pragma unroll
-                    for (int q = 0; q < 8; q++) cv[q] = (int)conj_swap_f8w((uint32_t)av[q]);
-#pragma unroll
-                    for (int j = 0; j < 2; j++) {
-                        const int t = 2 * i + j;
-                        if ((okm >> t) & 1u) {
-                            accr[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv[j], accr[t], 0, 0, 0,
-                                                                                     127, 0, 127);
-                            acci[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, bv[j], acci[t], 0, 0, 0,
-                                                                                     127, 0, 127);
-                        }
+                    if ((okm >> j) & 1u) {
+                        const uint4 b0 = lds[stg][sub][pb][4 * ks + 2 * kg][c0 + j * FX_BLK];
+                        const uint4 b1 = lds[stg][sub][pb][4 * ks + 2 * kg + 1][c0 + j * FX_BLK];
+                        const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
+                                          (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+                        accr[j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, accr[j], 0, 0, 0, 127, 0, 127);
+                        acci[j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, bv, acci[j], 0, 0, 0, 127, 0, 127);
                     }
                 }
             }
@@ -389,7 +385,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
         // so the result is the same on every run
         static_assert(sizeof(lds) >= 2 * 16 * 64 * sizeof(float), "staging LDS holds one wave's partial sums");
         float* red = reinterpret_cast<float*>(&lds[0][0][0][0][0]);
-        for (int w = 1; w < WAVES; w++) {
+        for (int w = 1; w < NW; w++) {
             if (wave == w) {
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
@@ -574,7 +570,10 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
     const dim3 grid(a.p.nchan * npairs), block(64 * WAVES);
     constexpr int DEEP = PREC == FX_PREC_F8 ? 2 : 4;    // 32 KB of LDS either way
     switch (SB) {
-    case 4: fx_xengine_kernel<PREC, 4, 1><<<grid, block, 0, s>>>(a); break;
+    case 4:
+        if constexpr (PREC == FX_PREC_F32) fx_xengine_kernel<PREC, 4, 1><<<grid, block, 0, s>>>(a);
+        else fx_xengine_kernel<PREC, 4, 1, 8><<<grid, dim3(64 * 8), 0, s>>>(a);
+        break;
     case 2: fx_xengine_kernel<PREC, 2, 1><<<grid, block, 0, s>>>(a); break;
     default:
         if ((long long)a.p.nchan * npairs <= kSmallGrid) fx_xengine_kernel<PREC, 1, DEEP><<<grid, block, 0, s>>>(a);
