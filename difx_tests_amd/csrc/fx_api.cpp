@@ -1154,11 +1154,15 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
     int bt = 0;
     for (int t = 1; t < 3; t++)
         if (xms[t] < xms[bt]) bt = t;
-    int best_tile = tiles[bt];
-    c->xsb = best_tile;
-    // subints per X-engine launch: whole steps of k F-stages and one X-engine
-    // launch over their slots; the smallest k within 1% of the best time per
-    // subint (a larger k only adds buffer memory and latency)
+    // tile and subints per X-engine launch together, in whole steps of k
+    // F-stages and one X-engine launch over their slots: a tile also changes
+    // the F-stages' speed (what the X-engine leaves in the caches, the power
+    // it draws), and a deeper tile needs more subints per launch to reach its
+    // steady state, neither of which timing the X-engine alone shows.  Tiles
+    // within 25% of the best alone, k = 1, 2, 4, ... (up to the FFT capacity),
+    // round-robin over the rounds with each pair's best kept; then the
+    // smallest k whose best tile is within 1% of the best time per subint (a
+    // larger k only adds buffer memory and latency) and that tile.
     auto step_ms = [&](int k, double* ms) {
         return tune_time(c, reps, ms, [&] {
             for (int s = 0; s < k; s++) {
@@ -1170,44 +1174,40 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
             return xstage_enqueue(c, 0, k * f.npad, c->stream);
         });
     };
-    double per_sub[8];
     int counts[8], nk = 0;
-    for (int k = 1; rc == FX_OK && nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) {
-        double ms = 0.0;
-        rc = step_ms(k, &ms);
-        counts[nk] = k;
-        per_sub[nk++] = ms / k;
-    }
-    int best_k = 1;
-    if (rc == FX_OK && nk > 0) {
-        double m = per_sub[0];
-        for (int i = 1; i < nk; i++) m = per_sub[i] < m ? per_sub[i] : m;
-        for (int i = 0; i < nk; i++)
-            if (per_sub[i] <= 1.01 * m) {
-                best_k = counts[i];
-                break;
+    for (int k = 1; nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) counts[nk++] = k;
+    double per_sub[3][8];
+    for (int t = 0; t < 3; t++)
+        for (int i = 0; i < 8; i++) per_sub[t][i] = 1e30;
+    for (int round = 0; rc == FX_OK && round < kRounds; round++)
+        for (int t = 0; rc == FX_OK && t < 3; t++) {
+            if (xms[t] > 1.25 * xms[bt]) continue;              // far slower alone (or skipped): not a candidate
+            c->xsb = tiles[t];
+            for (int i = 0; rc == FX_OK && i < nk; i++) {
+                double ms = 0.0;
+                rc = step_ms(counts[i], &ms);
+                if (rc == FX_OK) per_sub[t][i] = std::min(per_sub[t][i], ms / counts[i]);
             }
+        }
+    int best_k = 1, best_tile = tiles[bt];
+    if (rc == FX_OK && nk > 0) {
+        double m = 1e30;
+        for (int t = 0; t < 3; t++)
+            for (int i = 0; i < nk; i++) m = std::min(m, per_sub[t][i]);
+        bool found = false;
+        for (int i = 0; i < nk && !found; i++) {
+            int tb = -1;
+            for (int t = 0; t < 3; t++)
+                if (per_sub[t][i] <= 1.01 * m && (tb < 0 || per_sub[t][i] < per_sub[tb][i])) tb = t;
+            if (tb >= 0) {
+                best_k = counts[i];
+                best_tile = tiles[tb];
+                found = true;
+            }
+        }
     }
     c->xbatch = best_k;
-    // the X tile again, in whole steps of best_k subints: a tile also changes
-    // the F-stages' speed (what the X-engine leaves in the caches, the power
-    // it draws), which timing the X-engine alone does not show
-    {
-        double sms[3] = {1e30, 1e30, 1e30};
-        for (int round = 0; rc == FX_OK && round < kRounds; round++)
-            for (int t = 0; rc == FX_OK && t < 3; t++) {
-                if (xms[t] > 1.25 * xms[bt]) continue;          // far slower alone: not a candidate
-                c->xsb = tiles[t];
-                double ms = 0.0;
-                rc = step_ms(best_k, &ms);
-                if (rc == FX_OK) sms[t] = std::min(sms[t], ms);
-            }
-        int bs = bt;
-        for (int t = 0; t < 3; t++)
-            if (sms[t] < sms[bs]) bs = t;
-        best_tile = tiles[bs];
-        c->xsb = best_tile;
-    }
+    c->xsb = best_tile;
     const double best_x = xms[best_tile == 1 ? 0 : best_tile == 2 ? 1 : 2];
     c->timing = saved_timing;
     c->tap = saved_tap;

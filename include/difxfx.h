@@ -184,13 +184,14 @@ int fx_get_option(const fx_context *ctx, int option, int *value);
 /* Autotuner: times every F-stage radix plan, every X-engine tile and 1, 2,
  * 4, ... subints per X-engine launch (up to the FFT capacity) on this
  * context's uploaded input and delay model, numffts FFTs per subint, each
- * candidate once untimed then `reps` times (plans and tiles in three
- * round-robin rounds, best round kept); keeps the fastest of each (for the
- * launch count, timed as whole steps of that many F-stages and one X-engine
- * launch, the smallest within 3% of the best time per subint) and zeroes the
- * visibilities and counts.  report (may be NULL) receives {plan, tile,
- * subints per launch, F-stage ms per subint, X-engine ms per subint over one
- * subint}.  When
+ * candidate once untimed then `reps` times (in three round-robin rounds,
+ * best round kept); keeps the fastest plan, then times the tiles within 25%
+ * of the fastest tile alone together with each launch count as whole steps of
+ * that many F-stages and one X-engine launch, and keeps the smallest count
+ * whose best tile is within 1% of the best time per subint, with that tile;
+ * zeroes the visibilities and counts.  report (may be NULL) receives {plan,
+ * tile, subints per launch, F-stage ms per subint, X-engine ms per subint
+ * alone (up to 8 subints per launch)}.  When
  * cache_path (or, if it is NULL, the FX_TUNE_CACHE environment variable) names
  * a file, the result is stored there per (GPU arch, nant, nchan, prec), and
  * fx_create applies a stored entry when FX_TUNE_CACHE names that file. */
