@@ -239,11 +239,15 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
     }
 }
 
-// One wave per (FFT k of the call, antenna j), kDelayWaves per block: every
-// lane forms the delay entry (global FFT index iteration[k], or it0 + k when
-// iteration is null; buffer origin *origin_dev, or `origin` when that is
-// null), lane 0 writes it and the FFT slot's validity byte, and the wave fills
-// the rotator tables.  No barrier: the entry is the same in every lane.
+// One half-wave per (FFT k of the call, antenna j), 2 * kDelayWaves per
+// block: every lane forms the delay entry (global FFT index iteration[k], or
+// it0 + k when iteration is null; buffer origin *origin_dev, or `origin` when
+// that is null), the half-wave's first lane writes it and the FFT slot's
+// validity byte, and the half-wave fills the rotator tables.  No barrier: the
+// entry is the same in every lane of a half-wave.  Two entries per wave halve
+// the FP64 delay-model work per entry, and a table of 160 entries (1024
+// channels) takes five full 32-lane rounds instead of three 64-lane rounds
+// with the last one half idle.
 static constexpr int kDelayWaves = 4;
 __global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
     FxPlan p, const double* __restrict__ delays, const double* __restrict__ fileoff, const int* __restrict__ iteration,
@@ -251,8 +255,8 @@ __global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
     const float* __restrict__ subchf, const float* __restrict__ stepchf, int step_q, FxDelay* __restrict__ dtab,
     float2* __restrict__ rtab, uint8_t* __restrict__ valid_slots)
 {
-    const long long fa = (long long)blockIdx.x * kDelayWaves + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const long long fa = (long long)blockIdx.x * (2 * kDelayWaves) + (threadIdx.x >> 5);
+    const int lane = threadIdx.x & 31;
     if (fa >= nfa) return;
     const int k = (int)(fa / p.nant), j = (int)(fa - (long long)k * p.nant);
     FxDelay e;
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
         dtab[fa] = e;
         valid_slots[fa] = (uint8_t)e.valid;
     }
-    if (e.valid) rotator_fill(p, e, subchf, stepchf, step_q, rtab + fa * fx_rtab_len(p), lane, 64);
+    if (e.valid) rotator_fill(p, e, subchf, stepchf, step_q, rtab + fa * fx_rtab_len(p), lane, 32);
 }
 
 hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
@@ -278,7 +282,7 @@ hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const do
     if (n <= 0) return hipSuccess;
     if (step_q < 0 || (step_q & (step_q - 1)) != 0 || (p.nstep & (p.nstep - 1)) != 0 || step_q > p.nstep)
         return hipErrorInvalidValue;
-    fx_delay_rot_kernel<<<(unsigned)((n + kDelayWaves - 1) / kDelayWaves), 64 * kDelayWaves, 0, s>>>(
+    fx_delay_rot_kernel<<<(unsigned)((n + 2 * kDelayWaves - 1) / (2 * kDelayWaves)), 64 * kDelayWaves, 0, s>>>(
         p, delays, fileoff, iteration, it0, numffts, n, maxoffset, origin_dev, origin, subchf, stepchf, step_q, dtab,
         rtab, valid_slots);
     return hipGetLastError();
