@@ -34,6 +34,21 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 constexpr int NBW = 4;          // 32x32 blocks per wave
 constexpr int WAVES = 4;        // waves per workgroup
 
+typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+
+// 16-byte global load, non-temporal when NT (the line is not kept in the
+// caches for reuse)
+template <bool NT>
+__device__ __forceinline__ uint4 ld16(const uint4* p)
+{
+    if constexpr (NT) {
+        const u32x4v v = __builtin_nontemporal_load(reinterpret_cast<const u32x4v*>(p));
+        return make_uint4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
+}
+
 // (re, im) fp16 pairs -> (im, -re)
 __device__ __forceinline__ uint4 conj_swap_f16(uint4 x)
 {
@@ -106,7 +121,9 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // chunks: the global loads of step k+1 are in flight while the MFMAs of step
 // k run.  NSUB > 1 serves grids too small to fill the GPU (few channels and
 // blocks), whose workgroups would otherwise wait one memory latency per chunk.
-template <int PREC, int SB, int NSUB, int NW = WAVES>
+// NTL: the input is loaded non-temporally, for grids in which one super-tile
+// pair covers every signal, so that no other workgroup reads the same rows.
+template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
@@ -235,7 +252,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             const int chunk = sp * NSUB + sub;
             const int chc = chunk < nchunk ? chunk : nchunk - 1;
             if (NSUB > 1 || !(diag && pan))
-                st[l] = gbase[chc * cstride + g * Spad + (sig < Spad ? sig : Spad - 1)];
+                st[l] = ld16<NTL>(gbase + (chc * cstride + g * Spad + (sig < Spad ? sig : Spad - 1)));
         }
     };
     auto sstore = [&](int stage) {
@@ -571,8 +588,18 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
     constexpr int DEEP = PREC == FX_PREC_F8 ? 2 : 4;    // 32 KB of LDS either way
     switch (SB) {
     case 4:
-        if constexpr (PREC == FX_PREC_F32) fx_xengine_kernel<PREC, 4, 1><<<grid, block, 0, s>>>(a);
-        else fx_xengine_kernel<PREC, 4, 1, 8><<<grid, dim3(64 * 8), 0, s>>>(a);
+        // FP16 with one panel pair per channel (at most 64 antennas): every
+        // input row is read by exactly one workgroup, and non-temporal loads
+        // measured 5.7% faster there (64 x 4096); where panels are shared
+        // (larger arrays, smaller tiles) the reuse through the L2 is worth more
+        if constexpr (PREC == FX_PREC_F32) {
+            fx_xengine_kernel<PREC, 4, 1><<<grid, block, 0, s>>>(a);
+        } else if constexpr (PREC == FX_PREC_F16) {
+            if (npan == 1) fx_xengine_kernel<PREC, 4, 1, 8, true><<<grid, dim3(64 * 8), 0, s>>>(a);
+            else fx_xengine_kernel<PREC, 4, 1, 8><<<grid, dim3(64 * 8), 0, s>>>(a);
+        } else {
+            fx_xengine_kernel<PREC, 4, 1, 8><<<grid, dim3(64 * 8), 0, s>>>(a);
+        }
         break;
     case 2: fx_xengine_kernel<PREC, 2, 1><<<grid, block, 0, s>>>(a); break;
     default:
