@@ -106,9 +106,11 @@ def test_graph_replays_against_eager():
         e.close()
 
 
-@pytest.mark.parametrize("prec", ["f32", "f16", "f32v"])
-def test_xtile_does_not_change_results(prec):
-    nchan, nant, nf = 256, 70, 4                           # 5 signal blocks: partial last panels
+# 70 antennas: 5 signal blocks, partial last panels; 48 antennas: 3 blocks,
+# one panel pair per channel at tile 4 (the FP16 non-temporal-load kernel)
+@pytest.mark.parametrize("prec,nant", [("f32", 70), ("f16", 70), ("f32v", 70), ("f16", 48)])
+def test_xtile_does_not_change_results(prec, nant):
+    nchan, nf = 256, 4
     p, data, delays, fileoff = _subband(nchan, nant, 16e6, nf, 77)
     out = {}
     for sb in (1, 2, 4):
