@@ -29,6 +29,7 @@ namespace fx {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr int NBW = 4;          // 32x32 blocks per wave
@@ -108,6 +109,19 @@ __device__ __forceinline__ void store_block(const XArgs& a, int c, int bp, int I
     }
 }
 
+// The top-left 16x16 quarter of block (0, 0) of channel c from a 16x16 MFMA
+// accumulator: rows 4 * (lane / 16) + v, column lane % 16.
+__device__ __forceinline__ void store_block16(const XArgs& a, int c, int lane, const f32x4& re, const f32x4& im)
+{
+    const int n = lane & 15, m0 = 4 * (lane >> 4);
+    float2* dst = a.vacc + ((long long)c * a.p.nbp + fx_bp_index(0, 0, a.p.nblk)) * (FX_BLK * FX_BLK);
+    float2 old[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) old[v] = a.accumulate ? dst[(m0 + v) * FX_BLK + n] : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int v = 0; v < 4; v++) dst[(m0 + v) * FX_BLK + n] = make_float2(re[v] + old[v].x, im[v] + old[v].y);
+}
+
 // K-chunk staging: groups of G FFTs per LDS stage, per precision.
 template <int PREC> struct XChunk;
 template <> struct XChunk<FX_PREC_F32> { static constexpr int KC = 4; };   // 8 FFTs
@@ -123,7 +137,10 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // blocks), whose workgroups would otherwise wait one memory latency per chunk.
 // NTL: the input is loaded non-temporally, for grids in which one super-tile
 // pair covers every signal, so that no other workgroup reads the same rows.
-template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false>
+// H16 (FP32, SB = 1, at most 16 signals): the one block is formed as its
+// top-left 16x16 quarter on v_mfma_f32_16x16x4f32, a quarter of the MFMA
+// work of a 32x32 block for the same K.
+template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false, bool H16 = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
@@ -266,13 +283,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         }
     };
 
+    static_assert(!H16 || (PREC == FX_PREC_F32 && SB == 1), "16x16 blocks: FP32, one block per pair");
     f32x16 accr[NBW], acci[NBW];
 #pragma unroll
     for (int t = 0; t < NBW; t++) {
         accr[t] = (f32x16){};
         acci[t] = (f32x16){};
     }
+    f32x4 acr16 = {}, aci16 = {};
     const int row = lane & 31, kg = lane >> 5;
+    const int r16 = lane & 15, k4 = lane >> 4;
     const int pb = diag ? 0 : 1;                          // panel of the B operand
 
     if (nstep > 0) {
@@ -318,6 +338,23 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
                         acci[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(conj_swap_f16(A)), hb, acci[t], 0,
                                                                          0, 0);
                     }
+                }
+            }
+        } else if constexpr (PREC == FX_PREC_F32 && H16) {
+            // one K step per row (2 FFTs): lane l holds element l / 16 of row
+            // l % 16 in the order (re0, re1, im0, im1), and of A' = (im, -re)
+            // in the order (im0, im1, -re0, -re1)
+#pragma unroll
+            for (int g = 0; g < KC; g++) {
+                if (!mine((sp * NSUB + sub) * KC + g)) continue;
+                if (okm & 1u) {
+                    const float4 A = as_float4(lds[stg][sub][0][g][r16]);
+                    const float4 B = as_float4(lds[stg][sub][pb][g][r16]);
+                    const float av = k4 == 0 ? A.x : k4 == 1 ? A.y : k4 == 2 ? A.z : A.w;
+                    const float cv = k4 == 0 ? A.z : k4 == 1 ? A.w : k4 == 2 ? -A.x : -A.y;
+                    const float bv = k4 == 0 ? B.x : k4 == 1 ? B.y : k4 == 2 ? B.z : B.w;
+                    acr16 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acr16, 0, 0, 0);
+                    aci16 = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, bv, aci16, 0, 0, 0);
                 }
             }
         } else if constexpr (PREC == FX_PREC_F32) {
@@ -404,23 +441,43 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         float* red = reinterpret_cast<float*>(&lds[0][0][0][0][0]);
         for (int w = 1; w < NW; w++) {
             if (wave == w) {
+                if constexpr (H16) {
 #pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    red[v * 64 + lane] = accr[0][v];
-                    red[(16 + v) * 64 + lane] = acci[0][v];
+                    for (int v = 0; v < 4; v++) {
+                        red[v * 64 + lane] = acr16[v];
+                        red[(4 + v) * 64 + lane] = aci16[v];
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        red[v * 64 + lane] = accr[0][v];
+                        red[(16 + v) * 64 + lane] = acci[0][v];
+                    }
                 }
             }
             __syncthreads();
             if (wave == 0) {
+                if constexpr (H16) {
 #pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    accr[0][v] += red[v * 64 + lane];
-                    acci[0][v] += red[(16 + v) * 64 + lane];
+                    for (int v = 0; v < 4; v++) {
+                        acr16[v] += red[v * 64 + lane];
+                        aci16[v] += red[(4 + v) * 64 + lane];
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        accr[0][v] += red[v * 64 + lane];
+                        acci[0][v] += red[(16 + v) * 64 + lane];
+                    }
                 }
             }
             __syncthreads();
         }
         if (wave != 0) okm = 0;
+    }
+    if constexpr (H16) {
+        if (okm & 1u) store_block16(a, c, lane, acr16, aci16);
+        return;
     }
     const float* sc = (PREC == FX_PREC_F8) ? a.scale + (long long)c * Spad : nullptr;
 #pragma unroll
@@ -603,6 +660,15 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
         break;
     case 2: fx_xengine_kernel<PREC, 2, 1><<<grid, block, 0, s>>>(a); break;
     default:
+        if constexpr (PREC == FX_PREC_F32) {
+            if (a.p.S <= FX_BLK / 2) {                    // at most 16 signals: 16x16 MFMA blocks
+                if ((long long)a.p.nchan * npairs <= kSmallGrid)
+                    fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, true><<<grid, block, 0, s>>>(a);
+                else
+                    fx_xengine_kernel<PREC, 1, 1, WAVES, false, true><<<grid, block, 0, s>>>(a);
+                break;
+            }
+        }
         if ((long long)a.p.nchan * npairs <= kSmallGrid) fx_xengine_kernel<PREC, 1, DEEP><<<grid, block, 0, s>>>(a);
         else fx_xengine_kernel<PREC, 1, 1><<<grid, block, 0, s>>>(a);
         break;
