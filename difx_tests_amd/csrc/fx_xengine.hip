@@ -145,8 +145,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 {
     constexpr int KC = XChunk<PREC>::KC;
     constexpr int PW = SB * FX_BLK;                      // signals per panel
-    constexpr int NELEM = 2 * PW * KC;                   // 16-byte rows per chunk (both panels)
+    // rows staged per group and panel, and panels staged: H16 stages only the
+    // 16 rows its quarter block reads, of its one (diagonal) panel
+    constexpr int PWL = H16 ? FX_BLK / 2 : PW;
+    constexpr int NPL = H16 ? 1 : 2;
+    constexpr int NELEM = NPL * PWL * KC;                // 16-byte rows per chunk
     constexpr int NLD = (NSUB * NELEM + 64 * NW - 1) / (64 * NW);
+    constexpr bool PART = (NSUB * NELEM) % (64 * NW) != 0;   // threads past the step's rows load nothing
     __shared__ __align__(16) uint4 lds[2][NSUB][2][KC][PW];   // [stage][chunk][panel][group][signal]
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     // blocks beyond nblk, which okm excludes), and with one chunk per step a
     // diagonal workgroup skips its second panel (a workgroup- or wave-uniform
     // branch; that LDS is not read).
-    static_assert(NELEM % (64 * NW) == 0, "whole loads per thread");
+    static_assert(H16 || NELEM % (64 * NW) == 0, "whole loads per thread");
     uint4 st[NLD];
 #pragma unroll
     for (int l = 0; l < NLD; l++) st[l] = make_uint4(0, 0, 0, 0);
@@ -259,9 +264,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * NW;
+            if (PART && e >= NSUB * NELEM) continue;
             const int sub = e / NELEM, e2 = e - sub * NELEM;
-            const int pan = e2 / (KC * PW), rem = e2 - pan * (KC * PW);
-            const int g = rem / PW, sgl = rem - g * PW;
+            const int pan = NPL == 1 ? 0 : e2 / (KC * PWL), rem = e2 - pan * (KC * PWL);
+            const int g = rem / PWL, sgl = rem - g * PWL;
             const int sig = (pan ? SJ : SI) * PW + sgl;
             // the last step's chunks past nchunk re-read the last chunk; with
             // several chunks per step every load is issued, so that they go
@@ -276,9 +282,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * NW;
+            if (PART && e >= NSUB * NELEM) continue;
             const int sub = e / NELEM, e2 = e - sub * NELEM;
-            const int pan = e2 / (KC * PW), rem = e2 - pan * (KC * PW);
-            const int g = rem / PW, sgl = rem - g * PW;
+            const int pan = NPL == 1 ? 0 : e2 / (KC * PWL), rem = e2 - pan * (KC * PWL);
+            const int g = rem / PWL, sgl = rem - g * PWL;
             lds[stage][sub][pan][g][sgl] = st[l];
         }
     };
