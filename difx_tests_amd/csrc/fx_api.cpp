@@ -38,7 +38,7 @@ static int fail(int code, const char* fmt, ...)
 
 // Kernel timing: event pairs recorded around launches on the context's
 // stream without synchronising; fx_get_timing waits and sums them.
-enum { TM_DELAY, TM_FSTAGE, TM_COUNTS, TM_XSTAGE, TM_N };
+enum { TM_DELAY, TM_FSTAGE, TM_COUNTS, TM_XSTAGE, TM_GRAPH, TM_N };
 struct EvPair {
     hipEvent_t a, b;
     int kind;
@@ -79,7 +79,8 @@ struct fx_context {
     float2* d_cross;
     float2* d_autos;
     float2* d_tap;
-    int tap;
+    float* d_ltap;      // decoded levels of the tap (FX_TAP_LEVELS)
+    int tap;            // FX_TAP_* bits
     int last_nffts;     // entries in the delay table of the last fx_fstage
     int h_origin;       // host copy of the buffer origin being uploaded
     int xsb;            // X-engine super-block size (0: by the signal count)
@@ -90,7 +91,14 @@ struct fx_context {
     std::vector<EvPair> pool;      // free event pairs
     std::vector<EvPair> pending;   // recorded, not yet summed
     double ms[TM_N];
-    int nl[2];
+    int nl[3];          // launches: F-stage, X-engine, subint graph
+    // recorded subint graphs (fx_subint), one per call shape
+    struct SubGraph {
+        long long key[9];
+        hipGraphExec_t exec;
+    };
+    std::vector<SubGraph> graphs;
+    int use_graph = 1;  // FX_OPT_GRAPH
 };
 
 static int xin_row_bytes(int prec)
@@ -114,7 +122,7 @@ static void free_ctx(fx_context* c)
     if (c->cstream) (void)hipStreamDestroy(c->cstream);
     void* bufs[] = {c->d_data, c->d_data1, c->d_model, c->d_iter, c->d_dtab, c->d_valid, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
                     c->d_xin, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv,
-                    c->d_cross, c->d_autos, c->d_tap};
+                    c->d_cross, c->d_autos, c->d_tap, c->d_ltap};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto* v : {&c->pool, &c->pending})
@@ -122,6 +130,7 @@ static void free_ctx(fx_context* c)
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -134,6 +143,13 @@ extern "C" void fx_destroy(fx_context* ctx)
 }
 
 static void tune_apply_cached(fx_context* c);
+
+static int make_plan(const fx_params* prm, FxPlan* p);
+
+namespace fx {
+int plan_from_params(const fx_params* prm, FxPlan* p) { return make_plan(prm, p); }
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace fx
 
 // Host-side plan: kernel.c:13-39, generalised to power-of-two nchan.
 static int make_plan(const fx_params* prm, FxPlan* p)
@@ -401,6 +417,10 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
         if (hipMalloc((void**)&c->d_tap, sizeof(float2) * (size_t)c->max_ffts * p.nant * 2 * p.nchan) != hipSuccess)
             return fail(FX_ENOMEM, "tap allocation failed");
     }
+    if ((c->tap & FX_TAP_LEVELS) && !c->d_ltap) {
+        if (hipMalloc((void**)&c->d_ltap, sizeof(float) * (size_t)c->max_ffts * p.nant * 2 * p.F) != hipSuccess)
+            return fail(FX_ENOMEM, "level tap allocation failed");
+    }
     f->numffts = numffts;
     f->npad = npad;
     f->maxoffset = (int)maxoffset;
@@ -455,6 +475,7 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     a.ngroups = f.npad / c->G;
     a.scale_inv = c->d_scale_inv;
     a.tap = c->tap ? c->d_tap : nullptr;
+    a.ltap = (c->tap & FX_TAP_LEVELS) && !p.iscomplex ? c->d_ltap : nullptr;
     a.plan = c->fplan;
     TIMED(TM_FSTAGE, s, launch_fstage(a, c->fprec, s));
     if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], s));           // the slot may be overwritten
@@ -584,12 +605,82 @@ extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
     return xstage_enqueue(c, fft0, nslots, c->stream);
 }
 
+// One subint as one hipGraph launch (fx_subint).  The graph holds the delay /
+// rotator kernel, the F-stage and, on batch boundaries, the X-engine and the
+// counts; it is recorded by stream capture on the first call of a shape and
+// replayed after that.  FFT indices and the buffer origin are copied to the
+// device ahead of each launch and read there by the delay kernel, so one
+// recording serves every subint of a stream.  Waits on a streaming slot's
+// copy and the slot's release stay outside the graph.
+static constexpr size_t kMaxSubGraphs = 64;
+
+static int subint_graph(fx_context* c, const FStageCall& f, int xslots)
+{
+    const long long key[9] = {f.numffts, f.maxoffset, f.slot, f.fft0, xslots, c->tap, c->xsb, c->fplan, c->prec};
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+        if (std::equal(key, key + 9, g.key)) exec = g.exec;
+    if (!exec) {
+        if (c->graphs.size() >= kMaxSubGraphs) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+            c->graphs.clear();
+        }
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = 1;
+        int rc = fstage_enqueue(c, f, c->stream);
+        if (rc == FX_OK && xslots > 0) rc = xstage_enqueue(c, 0, xslots, c->stream);
+        c->capturing = 0;
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+        if (rc != FX_OK || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != FX_OK ? rc : fail(FX_EHIP, "subint graph capture failed: %s", hipGetErrorString(ee));
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(FX_EHIP, "subint graph instantiation failed: %s", hipGetErrorString(ie));
+        fx_context::SubGraph sg;
+        std::copy(key, key + 9, sg.key);
+        sg.exec = exec;
+        c->graphs.push_back(sg);
+    }
+    const bool ingest = c->ingest;
+    if (ingest) HIPCHK(hipStreamWaitEvent(c->stream, c->copied[f.slot], 0));
+    TIMED(TM_GRAPH, c->stream, hipGraphLaunch(exec, c->stream));
+    if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], c->stream));
+    if (c->timing) c->nl[2]++;
+    c->last_nffts = f.npad;
+    return FX_OK;
+}
+
+extern "C" int fx_subint(fx_context* c, int slot, int numffts, const int* iteration, int split, int origin, int fft0,
+                         int xslots, int* nslots)
+{
+    FStageCall f;
+    int rc = fstage_check(c, numffts, iteration, split, fft0, &f, origin, slot);
+    if (rc) return rc;
+    if (xslots < 0 || (xslots > 0 && (xslots % c->granule || xslots > c->max_ffts || xslots < fft0 + f.npad)))
+        return fail(FX_EINVAL, "xslots=%d (granule %d, capacity %d, must cover the subint's slots [%d,%d))", xslots,
+                    c->granule, c->max_ffts, fft0, fft0 + f.npad);
+    if (nslots) *nslots = f.npad;
+    if (!c->use_graph) {
+        rc = iter_upload(c, iteration, f, c->stream);
+        if (rc == FX_OK) rc = fstage_enqueue(c, f, c->stream);
+        if (rc == FX_OK && xslots > 0) rc = xstage_enqueue(c, 0, xslots, c->stream);
+        return rc;
+    }
+    rc = iter_upload(c, iteration, f, c->stream, true);
+    if (rc) return rc;
+    return subint_graph(c, f, xslots);
+}
+
 extern "C" int fx_process(fx_context* c, int numffts, const int* iteration, int split)
 {
-    int ns = 0;
-    int rc = fx_fstage(c, numffts, iteration, split, 0, &ns);
-    if (rc) return rc;
-    return fx_xstage(c, 0, ns);
+    if (!c || !iteration) return fail(FX_EINVAL, "null argument");
+    if (numffts < 1) return fail(FX_EINVAL, "numffts=%d", numffts);
+    const int npad = (numffts + c->granule - 1) / c->granule * c->granule;
+    return fx_subint(c, 0, numffts, iteration, split, 0, 0, npad, nullptr);
 }
 
 extern "C" int fx_reset(fx_context* c)
@@ -717,7 +808,9 @@ extern "C" int fx_read_delay_table(fx_context* c, int nffts, void* out)
 extern "C" int fx_enable_tap(fx_context* c, int enable)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
-    c->tap = enable ? 1 : 0;
+    if (enable & ~(FX_TAP_SPECTRA | FX_TAP_LEVELS)) return fail(FX_EINVAL, "unknown tap bits 0x%x", enable);
+    // the levels ride on the spectra tap's kernel variant
+    c->tap = enable ? (enable | FX_TAP_SPECTRA) : 0;
     return FX_OK;
 }
 
@@ -727,6 +820,18 @@ extern "C" int fx_read_tap(fx_context* c, int nffts, float* out)
     if (!c->d_tap) return fail(FX_EINVAL, "tap not enabled before fx_fstage");
     if (nffts < 0 || nffts > c->max_ffts) return fail(FX_EINVAL, "nffts=%d", nffts);
     HIPCHK(hipMemcpyAsync(out, c->d_tap, sizeof(float2) * (size_t)nffts * c->p.nant * 2 * c->p.nchan,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FX_OK;
+}
+
+extern "C" int fx_read_tap_levels(fx_context* c, int nffts, float* out)
+{
+    if (!c || !out) return fail(FX_EINVAL, "null argument");
+    if (c->p.iscomplex) return fail(FX_EUNSUPPORTED, "level tap: real 2-bit input only");
+    if (!c->d_ltap) return fail(FX_EINVAL, "level tap not enabled before fx_fstage");
+    if (nffts < 0 || nffts > c->max_ffts) return fail(FX_EINVAL, "nffts=%d", nffts);
+    HIPCHK(hipMemcpyAsync(out, c->d_ltap, sizeof(float) * (size_t)nffts * c->p.nant * 2 * c->p.F,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return FX_OK;
@@ -770,7 +875,7 @@ extern "C" int fx_reset_timing(fx_context* c)
     int rc = ev_collect(c);
     if (rc) return rc;
     for (double& v : c->ms) v = 0.0;
-    c->nl[0] = c->nl[1] = 0;
+    c->nl[0] = c->nl[1] = c->nl[2] = 0;
     return FX_OK;
 }
 
@@ -785,6 +890,16 @@ extern "C" int fx_get_timing(fx_context* c, double* ms4, int* nlaunch2)
         nlaunch2[0] = c->nl[0];
         nlaunch2[1] = c->nl[1];
     }
+    return FX_OK;
+}
+
+extern "C" int fx_get_graph_timing(fx_context* c, double* ms, int* nlaunch)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    int rc = ev_collect(c);
+    if (rc) return rc;
+    if (ms) *ms = c->ms[TM_GRAPH];
+    if (nlaunch) *nlaunch = c->nl[2];
     return FX_OK;
 }
 
@@ -818,6 +933,10 @@ extern "C" int fx_set_option(fx_context* c, int option, int value)
         if (value < 1) return fail(FX_EINVAL, "subints per X-engine launch must be >= 1");
         c->xbatch = value;
         return FX_OK;
+    case FX_OPT_GRAPH:
+        if (value != 0 && value != 1) return fail(FX_EINVAL, "FX_OPT_GRAPH is 0 or 1");
+        c->use_graph = value;
+        return FX_OK;
     default: return fail(FX_EINVAL, "unknown option %d", option);
     }
 }
@@ -830,6 +949,7 @@ extern "C" int fx_get_option(const fx_context* c, int option, int* value)
     case FX_OPT_FPLAN: *value = c->fplan; return FX_OK;
     case FX_OPT_NPLANS: *value = fstage_plans(c->p.lg); return FX_OK;
     case FX_OPT_XBATCH: *value = c->xbatch; return FX_OK;
+    case FX_OPT_GRAPH: *value = c->use_graph; return FX_OK;
     default: return fail(FX_EINVAL, "unknown option %d", option);
     }
 }
@@ -1104,18 +1224,51 @@ static int tune_time(fx_context* c, int reps, double* ms, Fn fn)
     return FX_OK;
 }
 
+static int autotune_run(fx_context* c, int numffts, int reps, double* report, TuneEntry* out);
+
 extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cache_path, double* report)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
     if (reps < 1) return fail(FX_EINVAL, "reps=%d", reps);
+    // options, timing and tap state are restored on every exit path, and the
+    // trial products never stay in the accumulators
+    const int saved_timing = c->timing, saved_tap = c->tap;
+    const int saved_plan = c->fplan, saved_xsb = c->xsb, saved_xbatch = c->xbatch;
+    c->timing = 0;
+    c->tap = 0;
+    TuneEntry e;
+    int rc = autotune_run(c, numffts, reps, report, &e);
+    c->timing = saved_timing;
+    c->tap = saved_tap;
+    if (rc) {
+        c->fplan = saved_plan;
+        c->xsb = saved_xsb;
+        c->xbatch = saved_xbatch;
+        const std::string msg = g_err;
+        (void)hipStreamSynchronize(c->stream);
+        (void)launch_zero(c->d_vacc, c->vacc_bytes, c->stream);
+        (void)launch_zero(c->d_counts, sizeof(int) * c->p.nant * c->p.nant, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        return fail(rc, "%s", msg.c_str());
+    }
+    // the trials accumulated into the visibilities: start clean
+    HIPCHK(launch_zero(c->d_vacc, c->vacc_bytes, c->stream));
+    HIPCHK(launch_zero(c->d_counts, sizeof(int) * c->p.nant * c->p.nant, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const char* path = cache_path ? cache_path : getenv("FX_TUNE_CACHE");
+    if (path && *path) return tune_write(path, e);
+    return FX_OK;
+}
+
+// The autotuner's trials; leaves the chosen plan, tile and launch count set
+// and the result in *out.  The caller restores state on failure.
+static int autotune_run(fx_context* c, int numffts, int reps, double* report, TuneEntry* out)
+{
     std::vector<int> it(numffts > 0 ? numffts : 1);
     for (int k = 0; k < numffts; k++) it[k] = k;
     FStageCall f;
     int rc = fstage_check(c, numffts, it.data(), 1, 0, &f);
     if (rc) return rc;
-    const int saved_timing = c->timing, saved_tap = c->tap;
-    c->timing = 0;
-    c->tap = 0;
     rc = iter_upload(c, it.data(), f, c->stream);
     if (rc) return rc;
 
@@ -1206,16 +1359,10 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
             }
         }
     }
+    if (rc) return rc;
     c->xbatch = best_k;
     c->xsb = best_tile;
     const double best_x = xms[best_tile == 1 ? 0 : best_tile == 2 ? 1 : 2];
-    c->timing = saved_timing;
-    c->tap = saved_tap;
-    if (rc) return rc;
-    // the trials accumulated into the visibilities: start clean
-    HIPCHK(launch_zero(c->d_vacc, c->vacc_bytes, c->stream));
-    HIPCHK(launch_zero(c->d_counts, sizeof(int) * c->p.nant * c->p.nant, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
     if (report) {
         report[0] = best_plan;
         report[1] = best_tile;
@@ -1223,10 +1370,6 @@ extern "C" int fx_autotune(fx_context* c, int numffts, int reps, const char* cac
         report[3] = best_f;
         report[4] = best_x;
     }
-    const char* path = cache_path ? cache_path : getenv("FX_TUNE_CACHE");
-    if (path && *path) {
-        TuneEntry e{device_arch(), c->p.nant, c->p.nchan, c->prec, best_plan, best_tile, best_k, best_f, best_x};
-        return tune_write(path, e);
-    }
+    *out = TuneEntry{device_arch(), c->p.nant, c->p.nchan, c->prec, best_plan, best_tile, best_k, best_f, best_x};
     return FX_OK;
 }

@@ -1080,6 +1080,8 @@ void fx_fstage_kernel(FStageArgs args)
                     // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
                     // fine rotator first, then one contracted complex product
                     const float l = lt[2 * bp[r * (T0 / 2)]];
+                    if (TAP && args.ltap)
+                        args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + i + r * T0] = l;
                     const float2 st = SPERM ? stv[r] : SSTEP ? load_const(spg + r * Q) : sp[r * Q];
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), st);
                 }
@@ -1368,6 +1370,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
 #pragma unroll
                 for (int r = 0; r < R0; r++) {
                     const float l = lt[2 * bp[r * T0]];
+                    if (TAP && args.ltap)
+                        args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + 2 * (i + r * T0) + h] = l;
                     x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (2 * T0 / SUB)]);
                 }
                 dft<R0>(x);
@@ -1427,7 +1431,7 @@ static hipError_t launch_eo(const FStageArgs& a, int prec, hipStream_t s)
     const int nblocks = 2 * a.ngroups * a.p.nant;
     if (nblocks <= 0) return hipSuccess;
     const dim3 grid(nblocks);
-    const bool tap = a.tap != nullptr;
+    const bool tap = a.tap != nullptr || a.ltap != nullptr;
     switch (prec) {
     case FX_PREC_F32: launch_eo_prec<V, FX_PREC_F32>(a, grid, tap, s); break;
     case FX_PREC_F16: launch_eo_prec<V, FX_PREC_F16>(a, grid, tap, s); break;
@@ -1576,7 +1580,7 @@ static hipError_t launch_lgv(const FStageArgs& a, int prec, hipStream_t s)
     const int nblocks = 2 * a.ngroups * a.p.nant;     // one block per polarisation
     if (nblocks <= 0) return hipSuccess;
     const dim3 grid(nblocks), block(Geo<LG, V>::NT);
-    const bool tap = a.tap != nullptr;
+    const bool tap = a.tap != nullptr || a.ltap != nullptr;
     switch (prec) {
     case FX_PREC_F32:
         if (tap) fx_fstage_kernel<LG, V, FX_PREC_F32, true, CPLX><<<grid, block, 0, s>>>(a);

@@ -189,7 +189,7 @@ void processAntennasAndBaselinePara(int* nant, int* numffts, int* fftchannels, d
                                     cf32*** visibilities, int* baselineCount, int* iteration, double* bw,
                                     cf32*** visibilities_out, int* baselineCount_out, int split)
 {
-    (void)unpacked; (void)sampletime; (void)substridesize; (void)stridesize; (void)fractionalLoFreq;
+    (void)sampletime; (void)substridesize; (void)stridesize; (void)fractionalLoFreq;
     if (*nbit != 2) die("Unsupported number of bits!!!");
     if (*fftchannels != 2 * *nchan) die("Error: fftchannels must be 2*nchan for real data");
     const int na = *nant, nc = *nchan, nf = *numffts;
@@ -204,8 +204,7 @@ void processAntennasAndBaselinePara(int* nant, int* numffts, int* fftchannels, d
         for (int i = 0; i < 3; i++) dl[3 * (size_t)j + i] = delays[j][i];
     if (fx_set_model(ctx, dl.data(), antFileOffsets)) die_fx("delay model");
     if (fx_upload_data(ctx, (const uint8_t* const*)inputData, nbytes)) die_fx("input upload");
-    if (fx_reset(ctx) || fx_enable_tap(ctx, 1) || fx_set_timing(ctx, 1) || fx_reset_timing(ctx))
-        die_fx("reset");
+    if (fx_reset(ctx) || fx_set_timing(ctx, 1) || fx_reset_timing(ctx)) die_fx("reset");
     if (fx_process(ctx, nf, iteration, split)) die_fx("processAntennasAndBaselinePara");
     std::vector<float> vis((size_t)(nbl > 0 ? nbl : 1) * 4 * nc * 2);
     std::vector<int> cnt((size_t)(nbl > 0 ? nbl : 1));
@@ -222,38 +221,38 @@ void processAntennasAndBaselinePara(int* nant, int* numffts, int* fftchannels, d
         }
         baselineCount[b] += cnt[b];
     }
-    // channelised / conjchannels keep the last valid spectrum of each antenna
+    // unpacked / channelised / conjchannels are left holding each antenna's
+    // last valid FFT (kernel.c:725-800): formed on the device for those FFTs
+    // only, from the delay table of the call (32 bytes per (FFT, antenna))
     {
-        const int nslots = (nf + fx_fft_granule(ctx) - 1) / fx_fft_granule(ctx) * fx_fft_granule(ctx);
-        std::vector<float> tap((size_t)nslots * na * 2 * nc * 2);
-        std::vector<unsigned char> dtab((size_t)nslots * na * 32);
-        if (fx_read_tap(ctx, nf, tap.data()) || fx_read_delay_table(ctx, nf, dtab.data())) die_fx("tap read");
-        for (int j = 0; j < na; j++) {
-            for (int k = nf - 1; k >= 0; k--) {
-                int valid;
-                memcpy(&valid, dtab.data() + ((size_t)k * na + j) * 32 + 28, sizeof(int));
-                if (!valid) continue;
-                for (int p = 0; p < 2; p++) {
-                    const float* s = tap.data() + (((size_t)k * na + j) * 2 + p) * nc * 2;
-                    for (int c = 0; c < nc; c++) {
-                        channelised[j][p][c].re = s[2 * c];
-                        channelised[j][p][c].im = s[2 * c + 1];
-                        conjchannels[j][p][c].re = s[2 * c];
-                        conjchannels[j][p][c].im = -s[2 * c + 1];
-                    }
+        std::vector<FxDelay> dtab((size_t)nf * na);
+        if (fx_read_delay_table(ctx, nf, dtab.data())) die_fx("delay table read");
+        std::vector<FxDelay> last((size_t)na);
+        std::vector<int> has((size_t)na, 0);
+        for (int j = 0; j < na; j++)
+            for (int k = nf - 1; k >= 0; k--)
+                if (dtab[(size_t)k * na + j].valid) {
+                    last[j] = dtab[(size_t)k * na + j];
+                    has[j] = 1;
+                    break;
                 }
-                break;
-            }
-        }
+        void* dptr = nullptr;
+        long long dstride = 0;
+        if (fx_device_data(ctx, &dptr, &dstride)) die_fx("device data");
+        int lg = 0;
+        while ((1 << lg) < nc) lg++;
+        const int strd = 1 << (lg / 2);
+        fxk_fill_antenna_buffers(static_cast<const uint8_t*>(dptr), dstride, last.data(), has.data(), na,
+                                 *fftchannels, nc, 2 * strd, strd, *lo, (*lo - (int)(*lo)) > TINY, *bw, unpacked,
+                                 channelised, conjchannels);
     }
-    double ms[4] = {0, 0, 0, 0};
-    fx_get_timing(ctx, ms, nullptr);
+    // the subint is one graph launch (station delays, unpack, fringe, FFT,
+    // fractional delay and the X-engine), timed as a whole
+    double gms = 0.0;
+    fx_get_graph_timing(ctx, &gms, nullptr);
     fx_set_timing(ctx, 0);
-    fx_enable_tap(ctx, 0);
     clock_gettime(CLOCK_MONOTONIC, &t1);
-    printf("Time taken station in ms : %f \n", ms[0]);
-    printf("Time taken unpack+fringe+fft+frac+conj in ms : %f \n", ms[1]);
-    printf("Time taken base in ms : %f \n", ms[2] + ms[3]);
+    printf("Time taken station+unpack+fringe+fft+frac+conj+base (one graph launch) in ms : %f \n", gms);
     printf("Time taken call (incl. transfers) in ms : %f \n",
            (t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) * 1e-6);
     memcpy(visibilities_out, visibilities, (size_t)nbl * sizeof(cf32**));   // kernel.c:825-826

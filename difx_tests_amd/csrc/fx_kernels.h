@@ -2,9 +2,15 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include "difxfx.h"
 #include "fx_common.h"
 
 namespace fx {
+
+// Plan of fx_params (checks them; the error is left in fx_last_error) and the
+// error message of a failing call (returns code).
+int plan_from_params(const fx_params* prm, FxPlan* p);
+int set_error(int code, const char* msg);
 
 struct FStageArgs {
     FxPlan p;
@@ -21,6 +27,8 @@ struct FStageArgs {
     int ngroups;                  // groups in this launch
     const float* scale_inv;       // FP8: [Spad][nchan] inverse scales (channel fastest: coalesced)
     float2* tap;                  // optional FP32 spectra [nffts][nant][2][nchan]
+    float* ltap;                  // optional decoded 2-bit levels [nffts][nant][2][F] of the
+                                  // valid FFTs (real input; only with tap)
     int plan;                     // radix plan, 0 .. fstage_plans(lg) - 1
 };
 

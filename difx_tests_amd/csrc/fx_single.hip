@@ -157,6 +157,64 @@ void d2h(void* h, const void* d, size_t n) { CK(hipMemcpy(h, d, n, hipMemcpyDevi
 
 }  // namespace
 
+// Per-antenna buffers of the reference's FFT loop (kernel.c:725-800) after
+// processAntennasAndBaselinePara: the loop overwrites unpacked[j],
+// channelised[j] and conjchannels[j] at every FFT in which antenna j is
+// valid, so the state it leaves is that of each antenna's last valid FFT.
+// Only those FFTs are formed here, on the device from the call's packed input
+// (d_data, antenna a at d_data + a*stride): unpacked and fringe-rotated
+// samples (F), the spectrum (F bins, the lower nchan frac-corrected) and the
+// conjugate of the lower nchan.  Antennas with no valid FFT keep their
+// buffers, as the reference's do.
+void fxk_fill_antenna_buffers(const uint8_t* d_data, long long stride, const FxDelay* last, const int* has,
+                              int nant, int F, int nchan, int sub, int stridesize, double lo, int fracLO, double bw,
+                              cf32*** unpacked, cf32*** channelised, cf32*** conjchannels)
+{
+    std::vector<int> ants;
+    for (int j = 0; j < nant; j++)
+        if (has[j]) ants.push_back(j);
+    if (ants.empty()) return;
+    const int nv = (int)ants.size();
+    const int lg = fx_ilog2_exact(F);
+    if (lg < 0 || !fx::fstage_supported(lg)) {
+        fprintf(stderr, "Error calling FFT");
+        exit(1);
+    }
+    std::vector<float2> tw(F);
+    for (int m = 0; m < F; m++) {
+        const double ang = -2.0 * kPi * (double)m / (double)F;
+        tw[m] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    DBuf u(sizeof(float2) * 2 * (size_t)F * nv), y(sizeof(float2) * 2 * (size_t)F * nv),
+        z(sizeof(float2) * 2 * (size_t)nchan * nv), dtw(sizeof(float2) * F);
+    h2d(dtw.p, tw.data(), sizeof(float2) * F);
+    for (int v = 0; v < nv; v++) {
+        const FxDelay& e = last[ants[v]];
+        float2* u0 = u.as<float2>() + (size_t)(2 * v) * F;
+        k_unpack<<<(F + 255) / 256, 256>>>(d_data + (size_t)ants[v] * stride + e.offset / 2, e.offset & 1, F, u0,
+                                           u0 + F);
+        k_fringe<<<(F + 255) / 256, 256>>>(u0, u0 + F, e.a, e.b, sub, stridesize, F, lo, fracLO);
+    }
+    CK(hipGetLastError());
+    CK(fx::launch_fft_c2c(u.as<float2>(), y.as<float2>(), dtw.as<float2>(), lg, 2 * nv, 0));
+    for (int v = 0; v < nv; v++) {
+        float2* y0 = y.as<float2>() + (size_t)(2 * v) * F;
+        float2* z0 = z.as<float2>() + (size_t)(2 * v) * nchan;
+        k_frac<<<(nchan + 255) / 256, 256>>>(y0, y0 + F, (float)last[ants[v]].fd, stridesize, nchan, bw);
+        k_conj<<<(nchan + 255) / 256, 256>>>(y0, z0, nchan);
+        k_conj<<<(nchan + 255) / 256, 256>>>(y0 + F, z0 + nchan, nchan);
+    }
+    CK(hipGetLastError());
+    for (int v = 0; v < nv; v++) {
+        const int j = ants[v];
+        for (int p = 0; p < 2; p++) {
+            d2h(unpacked[j][p], u.as<float2>() + (size_t)(2 * v + p) * F, sizeof(float2) * F);
+            d2h(channelised[j][p], y.as<float2>() + (size_t)(2 * v + p) * F, sizeof(float2) * F);
+            d2h(conjchannels[j][p], z.as<float2>() + (size_t)(2 * v + p) * nchan, sizeof(float2) * nchan);
+        }
+    }
+}
+
 int fxk_fftchannels(void) { return g_fftchannels; }
 double fxk_bandwidth(void) { return g_bw; }
 void fxk_set_geometry(int fftchannels, double bw)

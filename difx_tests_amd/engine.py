@@ -25,11 +25,12 @@ NATIVE_SYMBOLS = [
     "fx_set_timing", "fx_reset_timing", "fx_get_timing", "fx_get_geometry", "fx_set_option", "fx_get_option",
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
     "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait", "fx_read_vis_ex",
-    "fx_save_vis", "fx_batch_process_ex",
+    "fx_save_vis", "fx_batch_process_ex", "fx_read_tap_levels", "fx_subint", "fx_get_graph_timing",
 ]
 VIS_NORMALIZE = 1
+TAP_SPECTRA, TAP_LEVELS = 1, 2
 VIS_TEXT, VIS_BINARY = 0, 1
-OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH = 1, 2, 3, 4
+OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH, OPT_GRAPH = 1, 2, 3, 4, 5
 BATCH_LAUNCH, BATCH_GRAPH = 0, 1
 # Every symbol include/kernel.h declares (kernel.h:61-134 of the reference).
 KERNELH_SYMBOLS = [
@@ -84,6 +85,9 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_read_delay_table.argtypes = [P, I, P]
     lib.fx_enable_tap.argtypes = [P, I]
     lib.fx_read_tap.argtypes = [P, I, P]
+    lib.fx_read_tap_levels.argtypes = [P, I, P]
+    lib.fx_subint.argtypes = [P, I, I, P, I, I, I, I, C.POINTER(I)]
+    lib.fx_get_graph_timing.argtypes = [P, P, P]
     lib.fx_set_fp8_scales.argtypes = [P, P]
     lib.fx_sync.argtypes = [P]
     lib.fx_set_timing.argtypes = [P, I]
@@ -198,6 +202,26 @@ class FxEngine:
     def ingest_wait(self, slot: int) -> None:
         self._check(self.lib.fx_ingest_wait(self.h, slot))
 
+    def subint(self, iteration, slot: int = 0, split: int = 1, origin: int = 0, fft0: int = 0,
+               xslots: int = 0) -> int:
+        """One subint as one graph launch: delay table, F-stage at fft0 and,
+        when xslots > 0, the X-engine over slots [0, xslots)."""
+        it = np.ascontiguousarray(iteration, dtype=np.int32)
+        ns = C.c_int()
+        self._check(self.lib.fx_subint(self.h, slot, len(it), _ptr(it), split, origin, fft0, xslots, C.byref(ns)))
+        return ns.value
+
+    def set_graph(self, on: bool) -> None:
+        """fx_subint / fx_process as recorded graphs (default) or plain launches."""
+        self._check(self.lib.fx_set_option(self.h, OPT_GRAPH, 1 if on else 0))
+
+    def graph_timing(self):
+        """(milliseconds, launches) of fx_subint graph launches since reset_timing."""
+        ms = C.c_double()
+        n = C.c_int()
+        self._check(self.lib.fx_get_graph_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def xstage(self, fft0: int, nslots: int) -> None:
         self._check(self.lib.fx_xstage(self.h, fft0, nslots))
 
@@ -233,8 +257,17 @@ class FxEngine:
         self._check(self.lib.fx_read_delay_table(self.h, nffts, _ptr(out)))
         return out.reshape(nffts, self.nant)
 
-    def enable_tap(self, on: bool = True) -> None:
-        self._check(self.lib.fx_enable_tap(self.h, 1 if on else 0))
+    def enable_tap(self, on: bool = True, levels: bool = False) -> None:
+        """FP32 spectra of every F-stage FFT; with levels also the 2-bit levels
+        the fused F-stage decoded (real input)."""
+        mask = (TAP_SPECTRA | (TAP_LEVELS if levels else 0)) if on else 0
+        self._check(self.lib.fx_enable_tap(self.h, mask))
+
+    def read_tap_levels(self, nffts: int) -> np.ndarray:
+        """float32 [nffts, nant, 2, F] decoded levels of the valid FFTs."""
+        out = np.full((nffts, self.nant, 2, self.F), np.nan, np.float32)
+        self._check(self.lib.fx_read_tap_levels(self.h, nffts, _ptr(out)))
+        return out
 
     def read_tap(self, nffts: int) -> np.ndarray:
         out = np.zeros((nffts, self.nant, 2, self.nchan), np.complex64)

@@ -53,6 +53,22 @@ static void die(const char *msg, const char *arg)
     exit(1);
 }
 
+/* File name of subband `sb` from an antenna's FILE: the first "%d" is replaced
+ * by the subband index; any other '%' is rejected (the name is never used as a
+ * format string).  Exits when the result does not fit `cap` bytes. */
+static void subband_file(char *out, size_t cap, const char *pattern, int sb)
+{
+    const char *d = strstr(pattern, "%d");
+    for (const char *q = pattern; *q; q++)
+        if (*q == '%' && q != d) die("FILE may hold no '%' other than one %d:", pattern);
+    int n;
+    if (d)
+        n = snprintf(out, cap, "%.*s%d%s", (int)(d - pattern), pattern, sb, d + 2);
+    else
+        n = snprintf(out, cap, "%s", pattern);
+    if (n < 0 || (size_t)n >= cap) die("file name too long:", pattern);
+}
+
 static void check(int rc, const char *what)
 {
     if (rc) {
@@ -220,7 +236,7 @@ int main(int argc, char **argv)
             memcpy(d + 3 * j, c.delays + 3 * a, 3 * sizeof(double));
             o[j] = c.fileoffsets[a];
             char name[4096];
-            snprintf(name, sizeof(name), c.files[a], i);
+            subband_file(name, sizeof(name), c.files[a], i);
             if (!(fps[i][j] = fopen(name, "rb"))) die("cannot open input file", name);
             bufs[i][j] = malloc((size_t)bytes[i]);
         }

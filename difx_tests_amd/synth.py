@@ -13,6 +13,9 @@ the same call yields the same bytes on any machine.
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 THRESH = 0.98
@@ -62,6 +65,38 @@ def make_data(nant: int, nsamp: int, seed: int, delays_samples=None, rho_common:
                 v = s * np.float32(rho_common) + _rng(seed, 2, j, p).standard_normal(nsamp, dtype=np.float32)
             codes.append(quantise(v))
         out[j] = pack_codes(codes[0], codes[1])
+    return out
+
+
+def make_stream(nant: int, nchunks: int, chunk: int, seed: int, shifts, rho_common: float = 1.0 / 3.0,
+                nthreads: int = 0) -> np.ndarray:
+    """Packed [nant][nchunks*chunk/2] bytes of a long stream: the make_data
+    model per chunk of `chunk` samples, chunk c with its own common signal,
+    shifted for antenna j by shifts[c][j] samples (the integer net delay of
+    the delay model at that chunk), and its own noise.  Chunks are generated
+    in parallel threads (numpy's generators release the GIL), each keyed by
+    (seed, chunk, antenna, pol), so the bytes do not depend on the threads."""
+    assert chunk % 2 == 0
+    sh = np.asarray(shifts, np.int64).reshape(nchunks, nant)
+    out = np.empty((nant, nchunks * chunk // 2), np.uint8)
+    t = np.float32(THRESH * SIGMA)
+    rho = np.float32(rho_common)
+
+    def work(c: int) -> None:
+        margin = int(np.abs(sh[c]).max())
+        common = [_rng(seed, 21, c, p).standard_normal(chunk + 2 * margin, dtype=np.float32) for p in range(2)]
+        for j in range(nant):
+            codes = []
+            for p in range(2):
+                s = common[p][margin + sh[c, j]: margin + sh[c, j] + chunk]
+                v = s * rho + _rng(seed, 22, c * nant + j, p).standard_normal(chunk, dtype=np.float32)
+                # quantise(): the number of thresholds -t, 0, t at or below v
+                codes.append(((v >= -t).astype(np.uint8) + (v >= 0) + (v >= t)).astype(np.uint8))
+            out[j, c * chunk // 2: (c + 1) * chunk // 2] = pack_codes(codes[0], codes[1])
+
+    n = nthreads or min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max(1, n)) as ex:
+        list(ex.map(work, range(nchunks)))
     return out
 
 

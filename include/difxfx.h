@@ -110,8 +110,24 @@ int fx_ingest_wait(fx_context *ctx, int slot);
 /* X-engine over FFT slots [fft0, fft0 + nslots), accumulated into the
  * context's visibilities. */
 int fx_xstage(fx_context *ctx, int fft0, int nslots);
-/* fx_fstage at slot 0 followed by fx_xstage over the slots it wrote. */
+/* fx_fstage at slot 0 followed by fx_xstage over the slots it wrote, as one
+ * graph launch (fx_subint). */
 int fx_process(fx_context *ctx, int numffts, const int *iteration, int split);
+
+/* One subintegration as ONE hipGraph launch: the delay / rotator kernel and
+ * the F-stage of fx_fstage_ex(slot, numffts, iteration, split, origin, fft0)
+ * and, when xslots > 0, the X-engine and valid-FFT counts of
+ * fx_xstage(0, xslots) after them (the X-engine on batch boundaries: a caller
+ * batching k subints passes xslots = 0 for the first k-1 and the total slot
+ * count with the last).  The graph is recorded by stream capture on the first
+ * call of a shape (numffts, split, slot, fft0, xslots and the tuning options)
+ * and replayed after that; FFT indices and the origin are copied to the
+ * device ahead of each launch, so one recording serves every subint of a
+ * stream.  Results equal the plain launches bit for bit; with FX_OPT_GRAPH 0
+ * the same work goes out as plain launches.  The kernel.h entry point and
+ * fx_process use this.  *nslots (may be NULL) receives the slots written. */
+int fx_subint(fx_context *ctx, int slot, int numffts, const int *iteration, int split, int origin, int fft0,
+              int xslots, int *nslots);
 
 /* Zero the accumulated visibilities and counts. */
 int fx_reset(fx_context *ctx);
@@ -145,9 +161,17 @@ int fx_save_vis(fx_context *ctx, const char *path, int format, int flags);
  * fx_fstage call. */
 int fx_read_delay_table(fx_context *ctx, int nffts, void *out);
 /* When enabled, fx_fstage also writes FP32 spectra [fft][ant][pol][nchan]
- * (complex) of its FFTs; fx_read_tap copies nffts of them. */
+ * (complex) of its FFTs; fx_read_tap copies nffts of them.  enable is a mask
+ * of FX_TAP_SPECTRA and FX_TAP_LEVELS (levels imply spectra); FX_TAP_LEVELS
+ * also keeps the 2-bit levels the fused F-stage decoded for every sample of
+ * the valid FFTs (real input), [fft][ant][pol][F] floats, copied by
+ * fx_read_tap_levels (kernel.c:397-426 unpack, inside the fused kernel).
+ * Entries of invalid (FFT, antenna) pairs are left unwritten. */
+#define FX_TAP_SPECTRA 1
+#define FX_TAP_LEVELS 2
 int fx_enable_tap(fx_context *ctx, int enable);
 int fx_read_tap(fx_context *ctx, int nffts, float *out);
+int fx_read_tap_levels(fx_context *ctx, int nffts, float *out);
 
 /* FP8: per (channel, signal) scales, scales[c*2*nant + 2*a + pol]; the
  * stored value is spectrum/scale.  Defaults to the analytic 2-bit power. */
@@ -162,6 +186,9 @@ int fx_sync(fx_context *ctx);
 int fx_set_timing(fx_context *ctx, int enable);
 int fx_reset_timing(fx_context *ctx);
 int fx_get_timing(fx_context *ctx, double *ms4, int *nlaunch2);
+/* Graph launches of fx_subint are timed as a whole (events around each
+ * launch): summed milliseconds and launch count since fx_reset_timing. */
+int fx_get_graph_timing(fx_context *ctx, double *ms, int *nlaunch);
 
 /* Geometry used by the kernels (for reports): fills
  * {F, stride, sub, nstep, nfstep, S, Spad, nblk, nbp, G, granule}. */
@@ -173,11 +200,14 @@ int fx_get_geometry(const fx_context *ctx, int *out11);
  * FX_OPT_FPLAN   F-stage radix plan, 0 .. FX_OPT_NPLANS - 1
  * FX_OPT_NPLANS  (read only) number of radix plans at this FFT length
  * FX_OPT_XBATCH  subints per X-engine launch recommended to callers that
- *                batch fx_fstage calls before one fx_xstage (default 1) */
+ *                batch fx_fstage calls before one fx_xstage (default 1)
+ * FX_OPT_GRAPH   1 (default): fx_subint / fx_process launch recorded graphs;
+ *                0: the same kernels as plain launches */
 #define FX_OPT_XTILE 1
 #define FX_OPT_FPLAN 2
 #define FX_OPT_NPLANS 3
 #define FX_OPT_XBATCH 4
+#define FX_OPT_GRAPH 5
 int fx_set_option(fx_context *ctx, int option, int value);
 int fx_get_option(const fx_context *ctx, int option, int *value);
 

@@ -138,7 +138,41 @@ def test_kernelh_fine_grained_functions(tmp_path, nchan):
     vis = take("<c8", nbl * 4 * nchan).reshape(nbl, 4, nchan)
     cnt = take("<i4", nbl)
     visn = take("<c8", nbl * 4 * nchan).reshape(nbl, 4, nchan)
+    unp2 = take("<c8", nant * 2 * F).reshape(nant, 2, F)
+    chan2 = take("<c8", nant * 2 * F).reshape(nant, 2, F)
+    conj2 = take("<c8", nant * 2 * nchan).reshape(nant, 2, nchan)
+    vis2 = take("<c8", nbl * 4 * nchan).reshape(nbl, 4, nchan)
+    cnt2 = take("<i4", nbl)
     assert pos == len(buf)
+    # processAntennasAndBaselinePara over FFTs 0..numffts-1: visibilities and
+    # counts, and every antenna's buffers hold its last valid FFT
+    full = orc.process_f32(p, data, delays, fileoff, np.arange(numffts), want_chan=True)
+    assert np.array_equal(cnt2, full["count"])
+    live = cnt2 > 0
+    assert live.any()
+    assert np.linalg.norm(vis2[live] - full["vis"][live]) / np.linalg.norm(full["vis"][live]) < 1e-5
+    nlast = 0
+    for j in range(nant):
+        ks = np.nonzero(full["valid"][:, j])[0]
+        if len(ks) == 0:
+            assert not np.any(unp2[j]) and not np.any(chan2[j]) and not np.any(conj2[j])
+            continue
+        k = int(ks[-1])
+        nlast += 1
+        mj, aj, bj = orc.station_delay(delays[j], k)
+        vj, oj, fdj = orc.offset(mj, fileoff[j], p.Ts, k, F, (numffts - 1) * F)
+        assert vj and oj == full["offset"][k, j]
+        l0, l1 = orc.unpack(data[j], oj, F)
+        rot = orc.fringe_rotator(p, aj, bj)
+        for pol, lv in enumerate((l0, l1)):
+            u = lv.astype(np.complex128) * rot
+            assert np.max(np.abs(unp2[j, pol] - u)) < 1e-5
+            Y = np.fft.fft(unp2[j, pol].astype(np.complex128))
+            assert np.linalg.norm(chan2[j, pol, nchan:] - Y[nchan:]) / np.linalg.norm(Y[nchan:]) < 1e-5
+            want = full["chan"][k, j, pol]
+            assert np.linalg.norm(chan2[j, pol, :nchan] - want) / np.linalg.norm(want) < 1e-5
+            assert np.array_equal(conj2[j, pol], np.conj(chan2[j, pol, :nchan]))
+    assert nlast > 0
     # processAntennas leaves the last FFT (numffts - 1), maxoffset (numffts-1)*F
     last = numffts - 1
     ref = orc.process_f32(p, data, delays, fileoff, np.array([last]), split=numffts, want_chan=True)
