@@ -26,7 +26,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["fx_fstage.hip", "fx_xengine.hip", "fx_single.hip", "fx_api.cpp", "fx_kernelh.cpp"]
+HIP_SOURCES = ["fx_fstage.hip", "fx_xengine.hip", "fx_single.hip", "fx_api.cpp", "fx_kernelh.cpp", "fx_noalloc.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

@@ -26,6 +26,7 @@ NATIVE_SYMBOLS = [
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
     "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait", "fx_read_vis_ex",
     "fx_save_vis", "fx_batch_process_ex", "fx_read_tap_levels", "fx_subint", "fx_get_graph_timing",
+    "fx_process_noalloc",
 ]
 VIS_NORMALIZE = 1
 TAP_SPECTRA, TAP_LEVELS = 1, 2

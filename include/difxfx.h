@@ -129,6 +129,32 @@ int fx_process(fx_context *ctx, int numffts, const int *iteration, int split);
 int fx_subint(fx_context *ctx, int slot, int numffts, const int *iteration, int split, int origin, int fft0,
               int xslots, int *nslots);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  fx_process followed by fx_read_vis of one subint,
+ * without a context: every input and output is the caller's device memory
+ * (packed samples, antenna a at data + a*stride; delays [nant][3],
+ * fileoffsets [nant] and iteration [numffts] on the device), and every
+ * intermediate lives in one file-scope device array of FX_NOALLOC_SCRATCH_MB
+ * MiB per GPU, held until the process ends; nothing is allocated or freed.
+ * Outputs (any may be NULL) are overwritten with this call's unnormalised
+ * visibilities in fx_read_vis's layout: cross [nbl][4][nchan][2],
+ * cross_counts [nbl], autos [nant][4][nchan][2], auto_counts [nant].  The
+ * work is queued on `stream` (a hipStream_t; NULL is the default stream) and
+ * the call returns without waiting.  Default radix plan and X-engine tile;
+ * FP8 takes the analytic default scales.  data must be 16-byte aligned and
+ * stride a multiple of 16 of at least the subint's bytes + 16.  A workload
+ * whose accumulators and one granule of FFTs do not fit the array fails with
+ * FX_ENOMEM before any launch (the accumulators take 8 KiB per channel and
+ * pair of 32-signal blocks: 64 MiB holds up to 16 antennas at 4096 channels,
+ * 48 at 1024); larger FFT counts go through in chunks.  Every call on a GPU shares that GPU's array, which keeps the last
+ * call's contents: calls that overlap on one GPU (two streams, two threads)
+ * race. */
+#define FX_NOALLOC_SCRATCH_MB 64
+int fx_process_noalloc(const fx_params *prm, const uint8_t *data, long long stride, const double *delays,
+                       const double *fileoffsets, const int *iteration, int numffts, int split, float *cross,
+                       int *cross_counts, float *autos, int *auto_counts, void *stream);
+
 /* Zero the accumulated visibilities and counts. */
 int fx_reset(fx_context *ctx);
 
