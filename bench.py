@@ -8,15 +8,18 @@ batch of `--xbatch` subintegrations already resident in HBM: FP64 delay
 table, fused unpack/fringe/FFT/fractional-delay F-stage (one launch per
 subint), valid-FFT counts, and one X-engine launch accumulating the batch
 into the visibilities.  The data are synthetic (seeded 2-bit Gaussian noise
-with a 0.1-correlated common signal) and every subint of a batch reuses the
-same buffer.
+with a 0.1-correlated common signal): 16 consecutive subints of distinct
+samples (658 MB at the default workload, more than the 256 MB Infinity
+Cache), cycled through with advancing global FFT indices, so every subint of
+a step streams fresh input from HBM.
 
 Prints one JSON line.  `value` is input Gsamples/s (antennas x pols x
-samples processed per second).  The same line carries the X-engine TFLOPS,
-the plain FP32 GPU pipeline on the same workload, the roofline of the
-dominant kernel (HIP-event timings on the engine's stream inside the timed
-region) and a bounded single-thread CPU baseline (the oracle's FP32
-restatement of the reference path).
+samples processed per second) with plain launches timed by HIP events on the
+engine's stream.  The same line carries the X-engine TFLOPS, the same steps
+as one graph launch per subint, the plain FP32 GPU pipeline on the same
+workload, the roofline of the dominant kernel (HIP-event timings inside the
+timed region) and a bounded CPU baseline (the oracle's FP32 restatement of
+the reference path) on all usable host threads and on one.
 """
 from __future__ import annotations
 
@@ -55,7 +58,9 @@ def parse():
     ap.add_argument("--no-fp32-compare", action="store_true")
     ap.add_argument("--no-subbands", action="store_true", help="skip the eight-subband batch measurement")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-resident (PCIe-inclusive) measurement")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline work (s); 0 disables")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-launch measurements")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="target CPU baseline work (s), split between the all-threads and one-thread legs; 0 disables")
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three plain 512 MiB device copies (calibrates the HBM traffic counters)")
@@ -78,12 +83,19 @@ def copy_probe():
 
 
 MAX_XBATCH = 16
+NBUF = 16           # subints of distinct input held in HBM and cycled through
 
 
-def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune=True, plan=None, xtile=None):
+def run_pipeline(w, prec, data, delays, fileoff, nf, steps, warmup, xbatch, tune=True, plan=None, xtile=None,
+                 graph=False, timing=True):
     """Returns (seconds per step, kernel ms per step dict, launch counts, slots
-    per subint, tuning dict).  xbatch 0 takes the autotuner's choice."""
-    nf = len(it)
+    per subint, tuning dict).  The input buffer holds NBUF consecutive subints
+    of distinct data; subint u of the run reads subint u mod NBUF of it (global
+    FFT indices (u mod NBUF)*nf + k, buffer offsets of the whole buffer as the
+    reference's split does), so a step streams fresh samples.  graph: every
+    subint is one recorded graph launch (fx_subint) with the X-engine in the
+    step's last one; otherwise plain launches with HIP-event kernel timing.
+    xbatch 0 takes the autotuner's choice."""
     cap = (xbatch or MAX_XBATCH) * (nf + 64)
     e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec=prec, max_ffts=cap, max_bytes_per_ant=data.shape[1])
     try:
@@ -100,17 +112,25 @@ def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune
         tuned["xbatch_used"] = xbatch
         gran = e.granule
         slots = (nf + gran - 1) // gran * gran
+        its = [(r * nf + np.arange(nf)).astype(np.int32) for r in range(NBUF)]
+        u = [0]
 
         def step():
             for s in range(xbatch):
-                e.fstage(it, 1, s * slots)
-            e.xstage(0, xbatch * slots)
+                it = its[u[0] % NBUF]
+                u[0] += 1
+                if graph:
+                    e.subint(it, split=NBUF, fft0=s * slots, xslots=xbatch * slots if s == xbatch - 1 else 0)
+                else:
+                    e.fstage(it, NBUF, s * slots)
+            if not graph:
+                e.xstage(0, xbatch * slots)
 
         e.reset()
         for _ in range(warmup):
             step()
         e.sync()
-        e.set_timing(True)
+        e.set_timing(timing)
         e.reset_timing()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -118,8 +138,11 @@ def run_pipeline(w, prec, data, delays, fileoff, it, steps, warmup, xbatch, tune
         e.sync()
         t1 = time.perf_counter()
         ms, nl = e.timing()
+        gms, ng = e.graph_timing()
         e.set_timing(False)
         per_step = {k: v / steps for k, v in ms.items()}
+        if graph and timing:
+            per_step = {"graph": gms / steps}
         return (t1 - t0) / steps, per_step, nl, slots, tuned
     finally:
         e.close()
@@ -261,33 +284,75 @@ def measured_valu(workload, prec, kernel):
     return None, None
 
 
-def cpu_baseline(w, data, delays, fileoff, budget_s):
-    """Single-thread FP32 restatement (oracle) on a bounded FFT sample."""
+def _cpu_leg(w, data, delays, fileoff, budget_s, nthreads):
+    """The oracle's FP32 restatement of the reference path on a bounded FFT
+    sample, with nthreads host threads (F-stage over (FFT, antenna) pairs,
+    X-engine over baselines)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     p = orc.plan(w.nchan, w.lo, w.bw)
-    # probe with one FFT of every antenna, then size the sample to the budget
+    nbuf_ffts = (data.shape[1] * 2) // w.F - 1
+    # probe with four FFTs of every antenna (per-call costs such as clearing
+    # the visibilities included), then size the sample to the budget
+    n0 = min(4, w.nffts)
     t0 = time.perf_counter()
-    chan, valid = orc.fstage_f32(p, data, delays, fileoff, np.arange(1, dtype=np.int32), split=w.nffts)
-    orc.xstage_f32(chan, valid)
+    chan, valid = orc.fstage_f32(p, data, delays, fileoff, np.arange(n0, dtype=np.int32), split=nbuf_ffts // n0,
+                                 nthreads=nthreads)
+    orc.xstage_f32(chan, valid, nthreads=nthreads)
     t1 = time.perf_counter()
-    nsample = int(max(1, min(w.nffts, budget_s / max(t1 - t0, 1e-6))))
+    nsample = int(max(1, min(w.nffts, n0 * budget_s / max(t1 - t0, 1e-6))))
     it = np.arange(nsample, dtype=np.int32)
     t0 = time.perf_counter()
-    chan, valid = orc.fstage_f32(p, data, delays, fileoff, it, split=1 if nsample == w.nffts else w.nffts // nsample,
-                                 nthreads=1)
+    chan, valid = orc.fstage_f32(p, data, delays, fileoff, it, split=nbuf_ffts // nsample, nthreads=nthreads)
     t1 = time.perf_counter()
-    orc.xstage_f32(chan, valid, nthreads=1)
+    orc.xstage_f32(chan, valid, nthreads=nthreads)
     t2 = time.perf_counter()
     samples = w.nant * 2 * nsample * w.F
+    return samples / (t2 - t0) / 1e9, nsample, t1 - t0, t2 - t1
+
+
+def cpu_baseline(w, data, delays, fileoff, budget_s):
+    """The oracle's FP32 restatement (no IPP) timed on this host: all the
+    threads this process may use (capped at 16, the CPU share of one GPU on
+    the measurement boxes) and one thread."""
+    visible = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0))
+    nth = max(1, min(16, usable))
+    v_all, n_all, f_all, x_all = _cpu_leg(w, data, delays, fileoff, budget_s / 2, nth)
+    v_one, n_one, f_one, x_one = _cpu_leg(w, data, delays, fileoff, budget_s / 2, 1)
     return {
-        "value": round(samples / (t2 - t0) / 1e9, 6),
+        "value": round(v_all, 6),
         "unit": "Gsamples/s",
-        "cores": 1,
+        "cores": nth,
         "kind": "port",
-        "sample": f"{nsample} FFTs x {w.nant} antennas x 2 pols of the same subint "
-                  f"(F {t1 - t0:.2f} s + X {t2 - t1:.2f} s, single thread, oracle FP32 restatement, no IPP)",
+        "sample": f"{n_all} FFTs x {w.nant} antennas x 2 pols (F {f_all:.2f} s + X {x_all:.2f} s on {nth} threads; "
+                  f"oracle FP32 restatement of kernel.c:709-854, no IPP; host shows {visible} CPUs, "
+                  f"{usable} usable by this process)",
+        "single_thread": {"value": round(v_one, 6), "cores": 1,
+                          "sample": f"{n_one} FFTs x {w.nant} antennas x 2 pols (F {f_one:.2f} s + X {x_one:.2f} s)"},
+        "host_cpus_visible": visible,
+        "host_cpus_usable": usable,
     }
+
+
+def graph_vs_launch(steps, warmup, seed):
+    """4 antennas x 1024 channels FP32 (the reference's CPU-runnable case, where
+    launch gaps weigh most): plain launches against one graph launch per subint,
+    same tuned plan, tile and subints per X-engine launch."""
+    w = workloads.C1
+    data, delays, fileoff = workloads.make_stream_subints(w, seed, NBUF)
+    nf = w.nffts
+    out = {"workload": w.name, "subints_per_step": None}
+    sec_l, _, _, _, tuned = run_pipeline(w, "f32", data, delays, fileoff, nf, steps, warmup, 0, True, timing=False)
+    xb = tuned["xbatch_used"]
+    sec_g, _, _, _, _ = run_pipeline(w, "f32", data, delays, fileoff, nf, steps, warmup, xb, False,
+                                     tuned["plan_used"], tuned["xtile_used"], graph=True, timing=False)
+    out["subints_per_step"] = xb
+    sps = xb * w.samples_per_subint
+    out["launch"] = {"ms_per_step": round(sec_l * 1e3, 4), "gsamples_s": round(sps / sec_l / 1e9, 3)}
+    out["graph"] = {"ms_per_step": round(sec_g * 1e3, 4), "gsamples_s": round(sps / sec_g / 1e9, 3)}
+    out["graph_speedup"] = round(sec_l / sec_g, 3)
+    return out
 
 
 def main():
@@ -301,11 +366,11 @@ def main():
         sys.exit(2)
     w = workloads.WORKLOADS[a.workload]
     prec = a.prec or w.prec
-    data, delays, fileoff, it = workloads.make_subint(w, a.seed)
-    nf = len(it)
+    data, delays, fileoff = workloads.make_stream_subints(w, a.seed, NBUF)
+    nf = w.nffts
     tune = not (a.no_autotune or a.plan is not None or a.xtile is not None)
     xb_req = a.xbatch or (0 if tune else 8)
-    sec, kms, nl, slots, tuned = run_pipeline(w, prec, data, delays, fileoff, it, a.steps, a.warmup, xb_req, tune,
+    sec, kms, nl, slots, tuned = run_pipeline(w, prec, data, delays, fileoff, nf, a.steps, a.warmup, xb_req, tune,
                                               a.plan, a.xtile)
     xbatch = tuned["xbatch_used"]
     samples_per_step = xbatch * w.samples_per_subint
@@ -356,7 +421,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "antennas": w.nant, "pols": 2, "nchan": w.nchan, "fs_msps": w.fs / 1e6,
                    "subint_ms": w.subint_s * 1e3, "ffts_per_subint": nf, "subints_per_step": xbatch,
-                   "xengine_prec": prec, "parallelism": "single GPU"},
+                   "xengine_prec": prec, "parallelism": "single GPU",
+                   "input_in_hbm_bytes": int(data.nbytes), "distinct_subints_cycled": NBUF},
         "xengine_tflops": round(x_tflops, 2),
         "fstage_gbs": round(f_gbs, 1),
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kms.items()},
@@ -368,14 +434,22 @@ def main():
     if not a.no_fp32_compare and prec not in ("f32", "f32v"):
         # the plain FP32 GPU pipeline (vector-ALU X-engine) and the FP32 MFMA one
         for key, p32 in (("fp32_pipeline", "f32v"), ("fp32_mfma_pipeline", "f32")):
-            s32, k32, _, _, t32 = run_pipeline(w, p32, data, delays, fileoff, it, max(3, a.steps // 2), 1, xbatch,
+            s32, k32, _, _, t32 = run_pipeline(w, p32, data, delays, fileoff, nf, max(3, a.steps // 2), 1, xbatch,
                                                tune)
             v32 = samples_per_step / s32 / 1e9
             out[key] = {"xengine": p32, "value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
                         "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
                         "tuning": t32, "speedup_vs_this": round(value / v32, 3)}
+    if not a.no_graph:
+        # the same steps with every subint as one recorded graph launch
+        sg, _, _, _, _ = run_pipeline(w, prec, data, delays, fileoff, nf, a.steps, a.warmup, xbatch, False,
+                                      tuned["plan_used"], tuned["xtile_used"], graph=True, timing=False)
+        out["graph_launch"] = {"value": round(samples_per_step / sg / 1e9, 3), "ms_per_step": round(sg * 1e3, 4),
+                               "launches_per_subint": 1, "vs_plain_launches": round(sec / sg, 3)}
+        out["c1_graph_vs_launch"] = graph_vs_launch(max(10, a.steps), a.warmup, a.seed + 1)
     if not a.no_stream:
-        out["pcie_inclusive"] = streaming_ingest(w, prec, data, delays, fileoff, nf, tuned, xbatch,
+        sub = np.ascontiguousarray(data[:, : (nf + 1) * w.F // 2])
+        out["pcie_inclusive"] = streaming_ingest(w, prec, sub, delays, fileoff, nf, tuned, xbatch,
                                                  max(16, 2 * xbatch))
     if not a.no_subbands:
         out["subband_batch"] = subband_batch(max(5, a.steps), 2, a.seed, tune=tune)

@@ -82,7 +82,12 @@ struct fx_context {
     float* d_ltap;      // decoded levels of the tap (FX_TAP_LEVELS)
     int tap;            // FX_TAP_* bits
     int last_nffts;     // entries in the delay table of the last fx_fstage
-    int h_origin;       // host copy of the buffer origin being uploaded
+    // pinned ring of {origin, first FFT index, list flag} blocks copied to
+    // d_iter + max_ffts ahead of launches that read them from the device
+    int* h_ring;
+    hipEvent_t ring_ev[64];
+    unsigned char ring_used[64];
+    int ring_pos;
     int xsb;            // X-engine super-block size (0: by the signal count)
     int fplan;          // F-stage radix plan
     int xbatch;         // subints per X-engine launch recommended to callers (1 unless tuned)
@@ -99,7 +104,13 @@ struct fx_context {
     };
     std::vector<SubGraph> graphs;
     int use_graph = 1;  // FX_OPT_GRAPH
+    int autoscale = 0;  // FX_OPT_FP8_AUTOSCALE
 };
+
+static constexpr int kRing = 64;
+// FP8: one rms of each spectral component maps to this value (e4m3 holds up to
+// 448, so a single FFT's component saturates only beyond 28 rms)
+static constexpr float kFp8Target = 16.0f;
 
 static int xin_row_bytes(int prec)
 {
@@ -131,6 +142,9 @@ static void free_ctx(fx_context* c)
             (void)hipEventDestroy(e.b);
         }
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    for (int i = 0; i < kRing; i++)
+        if (c->ring_ev[i]) (void)hipEventDestroy(c->ring_ev[i]);
+    if (c->h_ring) (void)hipHostFree(c->h_ring);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -232,7 +246,16 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     }
     ALLOC(c->d_data, (size_t)c->stride * p.nant);
     ALLOC(c->d_model, sizeof(double) * 4 * p.nant);
-    ALLOC(c->d_iter, sizeof(int) * (c->max_ffts + 1));   // FFT indices, then the buffer origin
+    ALLOC(c->d_iter, sizeof(int) * (c->max_ffts + 4));   // FFT indices, then {origin, it0, list flag}
+    if (hipHostMalloc((void**)&c->h_ring, sizeof(int) * 4 * kRing, hipHostMallocDefault) != hipSuccess) {
+        free_ctx(c);
+        return fail(FX_ENOMEM, "pinned FFT-index ring failed");
+    }
+    for (int i = 0; i < kRing; i++)
+        if (hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming) != hipSuccess) {
+            free_ctx(c);
+            return fail(FX_EHIP, "hipEventCreate failed");
+        }
     ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_valid, (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * fx_rtab_len(p));
@@ -430,21 +453,32 @@ static int fstage_check(fx_context* c, int numffts, const int* iteration, int sp
     return FX_OK;
 }
 
-// FFT indices and the buffer origin to the device (ahead of the launches
-// that read them, on stream s).
 // FFT indices and buffer origin for the delay kernel.  A contiguous run of
-// indices (the usual case) travels as kernel arguments; otherwise, or when
-// the launch is recorded into a graph that must read each subint's values
-// from device memory (device = true), they are copied to d_iter.
+// indices (the usual case) travels as kernel arguments.  A launch recorded
+// into a graph must read each subint's values from device memory (device =
+// true): the block {origin, first index, list flag} at d_iter + max_ffts is
+// copied from a pinned ring slot (an asynchronous DMA, no host staging), and
+// only a non-contiguous index list is copied to d_iter itself.
 static int iter_upload(fx_context* c, const int* iteration, FStageCall& f, hipStream_t s, bool device = false)
 {
     f.it0 = iteration[0];
-    f.contig = !device;
-    for (int k = 1; k < f.numffts && f.contig; k++) f.contig = iteration[k] == f.it0 + k;
+    bool contig = true;
+    for (int k = 1; k < f.numffts && contig; k++) contig = iteration[k] == f.it0 + k;
+    f.contig = contig && !device;
     if (f.contig) return FX_OK;
-    HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * f.numffts, hipMemcpyHostToDevice, s));
-    c->h_origin = f.origin;
-    HIPCHK(hipMemcpyAsync(c->d_iter + c->max_ffts, &c->h_origin, sizeof(int), hipMemcpyHostToDevice, s));
+    if (!contig)
+        HIPCHK(hipMemcpyAsync(c->d_iter, iteration, sizeof(int) * f.numffts, hipMemcpyHostToDevice, s));
+    const int slot = c->ring_pos;
+    c->ring_pos = (c->ring_pos + 1) % kRing;
+    if (c->ring_used[slot]) HIPCHK(hipEventSynchronize(c->ring_ev[slot]));   // its last copy has been made
+    int* blk = c->h_ring + 4 * slot;
+    blk[0] = f.origin;
+    blk[1] = f.it0;
+    blk[2] = contig ? 0 : 1;
+    blk[3] = 0;
+    HIPCHK(hipMemcpyAsync(c->d_iter + c->max_ffts, blk, sizeof(int) * 3, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(c->ring_ev[slot], s));
+    c->ring_used[slot] = 1;
     return FX_OK;
 }
 
@@ -594,6 +628,9 @@ static int xstage_enqueue(fx_context* c, int fft0, int nslots, hipStream_t s)
     TIMED(TM_XSTAGE, s, launch_xengine(a, c->prec, s));
     // valid-FFT counts of the same slots (kernel.c:850)
     TIMED(TM_COUNTS, s, launch_counts(c->p.nant, nslots, c->d_valid + (size_t)fft0 * c->p.nant, c->d_counts, s));
+    // FP8: the next F-stages scale by the autocorrelations accumulated so far
+    if (c->autoscale && c->prec == FX_PREC_F8)
+        HIPCHK(launch_fp8_autoscale(c->p, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv, kFp8Target, s));
     if (c->timing && !c->capturing) c->nl[1]++;
     return FX_OK;
 }
@@ -616,7 +653,8 @@ static constexpr size_t kMaxSubGraphs = 64;
 
 static int subint_graph(fx_context* c, const FStageCall& f, int xslots)
 {
-    const long long key[9] = {f.numffts, f.maxoffset, f.slot, f.fft0, xslots, c->tap, c->xsb, c->fplan, c->prec};
+    const long long key[9] = {f.numffts, f.maxoffset, f.slot, f.fft0, xslots, c->tap, c->xsb, c->fplan,
+                              c->autoscale};
     hipGraphExec_t exec = nullptr;
     for (auto& g : c->graphs)
         if (std::equal(key, key + 9, g.key)) exec = g.exec;
@@ -855,6 +893,14 @@ extern "C" int fx_set_fp8_scales(fx_context* c, const float* scales)
     return FX_OK;
 }
 
+extern "C" int fx_update_fp8_scales(fx_context* c)
+{
+    if (!c) return fail(FX_EINVAL, "null argument");
+    if (c->prec != FX_PREC_F8) return fail(FX_EINVAL, "FP8 scales: context precision is not FX_PREC_F8");
+    HIPCHK(launch_fp8_autoscale(c->p, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv, kFp8Target, c->stream));
+    return FX_OK;
+}
+
 extern "C" int fx_sync(fx_context* c)
 {
     if (!c) return fail(FX_EINVAL, "null argument");
@@ -937,6 +983,11 @@ extern "C" int fx_set_option(fx_context* c, int option, int value)
         if (value != 0 && value != 1) return fail(FX_EINVAL, "FX_OPT_GRAPH is 0 or 1");
         c->use_graph = value;
         return FX_OK;
+    case FX_OPT_FP8_AUTOSCALE:
+        if (value != 0 && value != 1) return fail(FX_EINVAL, "FX_OPT_FP8_AUTOSCALE is 0 or 1");
+        if (value && c->prec != FX_PREC_F8) return fail(FX_EINVAL, "FX_OPT_FP8_AUTOSCALE needs FX_PREC_F8");
+        c->autoscale = value;
+        return FX_OK;
     default: return fail(FX_EINVAL, "unknown option %d", option);
     }
 }
@@ -950,6 +1001,7 @@ extern "C" int fx_get_option(const fx_context* c, int option, int* value)
     case FX_OPT_NPLANS: *value = fstage_plans(c->p.lg); return FX_OK;
     case FX_OPT_XBATCH: *value = c->xbatch; return FX_OK;
     case FX_OPT_GRAPH: *value = c->use_graph; return FX_OK;
+    case FX_OPT_FP8_AUTOSCALE: *value = c->autoscale; return FX_OK;
     default: return fail(FX_EINVAL, "unknown option %d", option);
     }
 }
@@ -1072,7 +1124,8 @@ extern "C" int fx_batch_process_ex(fx_batch* b, const int* numffts, const int* c
         int rc = fstage_check(b->ctx[i], numffts[i], iteration[i], split, 0, &calls[i], origins ? origins[i] : 0);
         if (rc) return fail(rc, "subband %d: %s", i, fx_last_error());
         key.push_back(numffts[i]);
-        key.push_back(((long long)b->ctx[i]->tap * 8 + b->ctx[i]->xsb) * 8 + b->ctx[i]->fplan);
+        key.push_back((((long long)b->ctx[i]->autoscale * 8 + b->ctx[i]->tap) * 8 + b->ctx[i]->xsb) * 8 +
+                      b->ctx[i]->fplan);
     }
     key.push_back(split);
     // s0 starts after the work already queued on every context's stream

@@ -240,9 +240,9 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
 }
 
 // One half-wave per (FFT k of the call, antenna j), 2 * kDelayWaves per
-// block: every lane forms the delay entry (global FFT index iteration[k], or
-// it0 + k when iteration is null; buffer origin *origin_dev, or `origin` when
-// that is null), the half-wave's first lane writes it and the FFT slot's
+// block: every lane forms the delay entry (global FFT index and buffer
+// origin as launch_delay_rotators describes), the half-wave's first lane
+// writes it and the FFT slot's
 // validity byte, and the half-wave fills the rotator tables.  No barrier: the
 // entry is the same in every lane of a half-wave.  Two entries per wave halve
 // the FP64 delay-model work per entry, and a table of 160 entries (1024
@@ -262,9 +262,13 @@ __global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
     FxDelay e;
     if (k >= numffts) {
         e.a = 0.0; e.b = 0.0; e.fd = 0.0; e.offset = 0; e.valid = 0;
+    } else if (origin_dev) {
+        // {origin, first index, list flag} in device memory (graph launches)
+        const int g = origin_dev[2] ? iteration[k] : origin_dev[1] + k;
+        e = delay_entry(p, delays, fileoff, j, g, origin_dev[0], maxoffset);
     } else {
         const int g = iteration ? iteration[k] : it0 + k;
-        e = delay_entry(p, delays, fileoff, j, g, origin_dev ? *origin_dev : origin, maxoffset);
+        e = delay_entry(p, delays, fileoff, j, g, origin, maxoffset);
     }
     if (lane == 0) {
         dtab[fa] = e;

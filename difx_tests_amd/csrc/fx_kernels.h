@@ -50,9 +50,11 @@ hipError_t launch_delay(const FxPlan& p, const double* delays, const double* fil
                         int numffts, int nffts_pad, int maxoffset, const int* origin_dev, FxDelay* out,
                         hipStream_t s);
 // Delay entries, FFT-slot validity bytes and rotator tables of one call in one
-// launch.  iteration: device FFT indices, or null for it0 + k; origin_dev:
-// device buffer origin, or null for `origin` (a recorded graph reads both from
-// device memory).  valid_slots: [nffts_pad][nant] bytes of the call's slots.
+// launch.  With origin_dev null: iteration is a device FFT-index list, or
+// null for it0 + k, and the buffer origin is `origin`.  Otherwise origin_dev
+// points to device {origin, first index, list flag}: FFT k is iteration[k]
+// when the flag is set, else first index + k (a recorded graph reads every
+// subint's values from device memory).  valid_slots: [nffts_pad][nant] bytes of the call's slots.
 // step_q: order of the tables' second copy of the fringe steps
 // (fstage_step_q; 0 leaves it unwritten).
 hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
@@ -82,5 +84,10 @@ hipError_t launch_counts(int nant, int nslots, const uint8_t* valid_slots, int* 
 hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, float2* autos, const int* counts,
                          hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
+// FP8 scales [nchan][Spad] and inverse scales [Spad][nchan] from the
+// accumulated autocorrelations and valid-FFT counts (rms per component ->
+// target).
+hipError_t launch_fp8_autoscale(const FxPlan& p, const float2* vacc, const int* counts, float* scale,
+                                float* scale_inv, float target, hipStream_t s);
 
 }  // namespace fx

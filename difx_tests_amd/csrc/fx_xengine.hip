@@ -812,6 +812,39 @@ hipError_t launch_export(const FxPlan& p, const float2* vacc, float2* cross, flo
     return hipGetLastError();
 }
 
+// FP8 scales from the autocorrelations accumulated so far: per (channel c,
+// signal s) the mean parallel-hand auto power P / n over the antenna's valid
+// FFTs gives the rms of each spectral component, sqrt(P / 2n), which maps to
+// `target` (scale = rms / target, stored value = spectrum / scale).  Signals
+// without a valid FFT or power keep their scale.  vacc holds true units (the
+// X-engine applies s_i s_j before accumulating), so scales may change between
+// X-engine launches.
+__global__ void fx_fp8_autoscale_kernel(FxPlan p, const float2* __restrict__ vacc, const int* __restrict__ counts,
+                                        float* __restrict__ scale, float* __restrict__ scale_inv, float target)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)p.nchan * p.S) return;
+    const int c = (int)(t / p.S), sg = (int)(t - (long long)c * p.S);
+    const int n = counts[(sg >> 1) * p.nant + (sg >> 1)];
+    const int I = sg / FX_BLK, r = sg % FX_BLK;
+    const float P = vacc[((long long)c * p.nbp + fx_bp_index(I, I, p.nblk)) * (FX_BLK * FX_BLK) + r * (FX_BLK + 1)].x;
+    if (n <= 0 || !(P > 0.0f) || !(P < 3.0e38f)) return;
+    const float rms = sqrtf(P / (2.0f * (float)n));
+    const float sc = rms / target;
+    if (!(sc > 0.0f) || !(sc < 3.0e38f)) return;
+    scale[(long long)c * p.Spad + sg] = sc;
+    scale_inv[(long long)sg * p.nchan + c] = 1.0f / sc;
+}
+
+hipError_t launch_fp8_autoscale(const FxPlan& p, const float2* vacc, const int* counts, float* scale,
+                                float* scale_inv, float target, hipStream_t s)
+{
+    const long long n = (long long)p.nchan * p.S;
+    if (n <= 0) return hipSuccess;
+    fx_fp8_autoscale_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, vacc, counts, scale, scale_inv, target);
+    return hipGetLastError();
+}
+
 // Fill with zeros (a kernel, so graph capture records no memset node).
 __global__ void fx_zero_kernel(uint4* p, long long n16, unsigned char* tail, int ntail)
 {

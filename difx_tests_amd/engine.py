@@ -26,12 +26,12 @@ NATIVE_SYMBOLS = [
     "fx_batch_create", "fx_batch_destroy", "fx_batch_process", "fx_batch_sync", "fx_batch_captures",
     "fx_autotune", "fx_fstage_ex", "fx_ingest_host", "fx_ingest_submit", "fx_ingest_wait", "fx_read_vis_ex",
     "fx_save_vis", "fx_batch_process_ex", "fx_read_tap_levels", "fx_subint", "fx_get_graph_timing",
-    "fx_process_noalloc",
+    "fx_process_noalloc", "fx_update_fp8_scales",
 ]
 VIS_NORMALIZE = 1
 TAP_SPECTRA, TAP_LEVELS = 1, 2
 VIS_TEXT, VIS_BINARY = 0, 1
-OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH, OPT_GRAPH = 1, 2, 3, 4, 5
+OPT_XTILE, OPT_FPLAN, OPT_NPLANS, OPT_XBATCH, OPT_GRAPH, OPT_FP8_AUTOSCALE = 1, 2, 3, 4, 5, 6
 BATCH_LAUNCH, BATCH_GRAPH = 0, 1
 # Every symbol include/kernel.h declares (kernel.h:61-134 of the reference).
 KERNELH_SYMBOLS = [
@@ -90,6 +90,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.fx_subint.argtypes = [P, I, I, P, I, I, I, I, C.POINTER(I)]
     lib.fx_get_graph_timing.argtypes = [P, P, P]
     lib.fx_set_fp8_scales.argtypes = [P, P]
+    lib.fx_update_fp8_scales.argtypes = [P]
     lib.fx_sync.argtypes = [P]
     lib.fx_set_timing.argtypes = [P, I]
     lib.fx_reset_timing.argtypes = [P]
@@ -278,6 +279,14 @@ class FxEngine:
     def set_fp8_scales(self, scales: np.ndarray) -> None:
         s = np.ascontiguousarray(scales, dtype=np.float32).reshape(self.nchan, 2 * self.nant)
         self._check(self.lib.fx_set_fp8_scales(self.h, _ptr(s)))
+
+    def update_fp8_scales(self) -> None:
+        """FP8 scales from the autocorrelations accumulated since reset()."""
+        self._check(self.lib.fx_update_fp8_scales(self.h))
+
+    def set_fp8_autoscale(self, on: bool = True) -> None:
+        """FP8 scales follow the accumulated autocorrelations after every X-engine launch."""
+        self._check(self.lib.fx_set_option(self.h, OPT_FP8_AUTOSCALE, 1 if on else 0))
 
     # ---- tuning
     def _get(self, opt: int) -> int:

@@ -108,6 +108,20 @@ def make_subint(w: Workload, seed: int, nffts: int | None = None):
     return data, delays, fileoff, np.arange(nf, dtype=np.int32)
 
 
+def make_stream_subints(w: Workload, seed: int, nsub: int):
+    """(data uint8 [nant][bytes], delays, fileoffsets) of nsub consecutive
+    subints in one buffer (nsub * nffts FFTs plus one more FFT of samples):
+    subint r is FFTs r*nffts ... r*nffts + nffts - 1, global indices from 0,
+    each with its own correlated data (the common signal at the model's
+    integer net delay in the middle of that subint)."""
+    nf = w.nffts
+    delays, fileoff = delay_model(w, seed)
+    shifts = np.stack([np.round(net_delay_samples(w, delays, fileoff, r * nf + nf / 2.0)).astype(np.int64)
+                       for r in range(nsub + 1)])
+    data = synth.make_stream(w.nant, nsub + 1, nf * w.F, seed, shifts)
+    return np.ascontiguousarray(data[:, : (nsub * nf + 1) * w.F // 2]), delays, fileoff
+
+
 # The benchmark workloads: plain FP32 at 4 antennas, FP32 at 16 antennas with
 # fringe rates and fractional delays, FP16 at 64 antennas, FP8 at 256 antennas.
 C1 = Workload("c1-4ant-1024ch", nant=4, nchan=1024, fs=128e6, rate_max=1e-9)

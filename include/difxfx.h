@@ -200,8 +200,21 @@ int fx_read_tap(fx_context *ctx, int nffts, float *out);
 int fx_read_tap_levels(fx_context *ctx, int nffts, float *out);
 
 /* FP8: per (channel, signal) scales, scales[c*2*nant + 2*a + pol]; the
- * stored value is spectrum/scale.  Defaults to the analytic 2-bit power. */
+ * stored value is spectrum/scale.  Defaults to the analytic 2-bit power of a
+ * flat spectrum (one rms per component -> 16).  fx_update_fp8_scales sets
+ * them on the device from the autocorrelations accumulated since fx_reset:
+ * per (channel, signal) the rms of each component, sqrt(P / 2n) for auto
+ * power P over n valid FFTs, maps to 16 (signals without a valid FFT keep
+ * their scale).  With FX_OPT_FP8_AUTOSCALE 1 this follows every X-engine
+ * launch, so each batch of F-stages is scaled by the autocorrelations of the
+ * subints before it (a sloped bandpass or a strong narrow-band tone no longer
+ * saturates e4m3 at 448 after the first batch of an integration; scales carry
+ * over fx_reset into the next integration).  The X-engine applies s_i s_j
+ * before accumulating, so scales may change between its launches; a caller
+ * must not run F-stages of the next batch before the X-engine of the
+ * current one. */
 int fx_set_fp8_scales(fx_context *ctx, const float *scales);
+int fx_update_fp8_scales(fx_context *ctx);
 
 int fx_sync(fx_context *ctx);
 
@@ -220,7 +233,8 @@ int fx_get_graph_timing(fx_context *ctx, double *ms, int *nlaunch);
  * {F, stride, sub, nstep, nfstep, S, Spad, nblk, nbp, G, granule}. */
 int fx_get_geometry(const fx_context *ctx, int *out11);
 
-/* Tuning options (the autotuner sets these; results do not depend on them).
+/* Options.  The tuning options (the autotuner sets them) leave results
+ * unchanged; FX_OPT_FP8_AUTOSCALE does not.
  * FX_OPT_XTILE   X-engine workgroup tile, a square of 1, 2 or 4 blocks of
  *                32x32 signals per panel; 0 selects by the signal count
  * FX_OPT_FPLAN   F-stage radix plan, 0 .. FX_OPT_NPLANS - 1
@@ -228,12 +242,17 @@ int fx_get_geometry(const fx_context *ctx, int *out11);
  * FX_OPT_XBATCH  subints per X-engine launch recommended to callers that
  *                batch fx_fstage calls before one fx_xstage (default 1)
  * FX_OPT_GRAPH   1 (default): fx_subint / fx_process launch recorded graphs;
- *                0: the same kernels as plain launches */
+ *                0: the same kernels as plain launches
+ * FX_OPT_FP8_AUTOSCALE  1: FP8 scales follow the accumulated
+ *                autocorrelations after every X-engine launch (see
+ *                fx_update_fp8_scales); 0 (default): fixed scales.  FP8 only;
+ *                unlike the others this option changes results */
 #define FX_OPT_XTILE 1
 #define FX_OPT_FPLAN 2
 #define FX_OPT_NPLANS 3
 #define FX_OPT_XBATCH 4
 #define FX_OPT_GRAPH 5
+#define FX_OPT_FP8_AUTOSCALE 6
 int fx_set_option(fx_context *ctx, int option, int value);
 int fx_get_option(const fx_context *ctx, int option, int *value);
 
