@@ -97,12 +97,20 @@ struct fx_context {
     std::vector<EvPair> pending;   // recorded, not yet summed
     double ms[TM_N];
     int nl[3];          // launches: F-stage, X-engine, subint graph
-    // recorded subint graphs (fx_subint), one per call shape
+    // recorded subint graphs (fx_subint), one per call shape, each
+    // instantiated kExecCopies times
     struct SubGraph {
-        long long key[9];
-        hipGraphExec_t exec;
+        long long key[10];
+        hipGraph_t graph;          // kept: its delay-kernel node addresses argument updates
+        hipGraphNode_t dnode;      // delay / rotator kernel node (arguments updated per launch), or null
+        hipKernelNodeParams kp;    // its recorded parameters
+        hipGraphExec_t exec[8];
+        hipEvent_t done[8];        // each copy's last launch
+        unsigned char used[8];
+        int next;
     };
     std::vector<SubGraph> graphs;
+    int graph_args = 1;  // recorded graphs take contiguous FFT indices as updated kernel arguments
     int use_graph = 1;  // FX_OPT_GRAPH
     int autoscale = 0;  // FX_OPT_FP8_AUTOSCALE
 };
@@ -111,6 +119,20 @@ static constexpr int kRing = 64;
 // FP8: one rms of each spectral component maps to this value (e4m3 holds up to
 // 448, so a single FFT's component saturates only beyond 28 rms)
 static constexpr float kFp8Target = 16.0f;
+
+static constexpr int kExecCopies = 8;
+static_assert(sizeof(((fx_context::SubGraph*)nullptr)->exec) == kExecCopies * sizeof(hipGraphExec_t),
+              "SubGraph holds kExecCopies executable graphs");
+
+static void subgraph_free(fx_context::SubGraph& g)
+{
+    for (int i = 0; i < kExecCopies; i++) {
+        if (g.exec[i]) (void)hipGraphExecDestroy(g.exec[i]);
+        if (g.done[i]) (void)hipEventDestroy(g.done[i]);
+    }
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = fx_context::SubGraph{};
+}
 
 static int xin_row_bytes(int prec)
 {
@@ -141,7 +163,7 @@ static void free_ctx(fx_context* c)
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
-    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    for (auto& g : c->graphs) subgraph_free(g);
     for (int i = 0; i < kRing; i++)
         if (c->ring_ev[i]) (void)hipEventDestroy(c->ring_ev[i]);
     if (c->h_ring) (void)hipHostFree(c->h_ring);
@@ -645,51 +667,123 @@ extern "C" int fx_xstage(fx_context* c, int fft0, int nslots)
 // One subint as one hipGraph launch (fx_subint).  The graph holds the delay /
 // rotator kernel, the F-stage and, on batch boundaries, the X-engine and the
 // counts; it is recorded by stream capture on the first call of a shape and
-// replayed after that.  FFT indices and the buffer origin are copied to the
-// device ahead of each launch and read there by the delay kernel, so one
-// recording serves every subint of a stream.  Waits on a streaming slot's
+// replayed after that.  A contiguous run of FFT indices (the usual case)
+// reaches the delay kernel as two kernel arguments, the first index and the
+// buffer origin, which each launch updates in its executable graph
+// (hipGraphExecKernelNodeSetParams: no copy, no extra GPU command).  Every
+// shape is instantiated kExecCopies times and used round robin, and a copy's
+// arguments are updated only after its previous launch has completed, so an
+// update never meets a launch still queued.  Other index lists are copied to
+// the device ahead of the launch and read there.  Waits on a streaming slot's
 // copy and the slot's release stay outside the graph.
 static constexpr size_t kMaxSubGraphs = 64;
 
-static int subint_graph(fx_context* c, const FStageCall& f, int xslots)
+static int subgraph_record(fx_context* c, const FStageCall& f, int xslots, bool args, fx_context::SubGraph& sg)
 {
-    const long long key[9] = {f.numffts, f.maxoffset, f.slot, f.fft0, xslots, c->tap, c->xsb, c->fplan,
-                              c->autoscale};
-    hipGraphExec_t exec = nullptr;
-    for (auto& g : c->graphs)
-        if (std::equal(key, key + 9, g.key)) exec = g.exec;
-    if (!exec) {
-        if (c->graphs.size() >= kMaxSubGraphs) {
-            HIPCHK(hipStreamSynchronize(c->stream));
-            for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-            c->graphs.clear();
-        }
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        c->capturing = 1;
-        int rc = fstage_enqueue(c, f, c->stream);
-        if (rc == FX_OK && xslots > 0) rc = xstage_enqueue(c, 0, xslots, c->stream);
-        c->capturing = 0;
-        hipGraph_t g = nullptr;
-        const hipError_t ee = hipStreamEndCapture(c->stream, &g);
-        if (rc != FX_OK || ee != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc != FX_OK ? rc : fail(FX_EHIP, "subint graph capture failed: %s", hipGetErrorString(ee));
-        }
-        const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) return fail(FX_EHIP, "subint graph instantiation failed: %s", hipGetErrorString(ie));
-        fx_context::SubGraph sg;
-        std::copy(key, key + 9, sg.key);
-        sg.exec = exec;
-        c->graphs.push_back(sg);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = 1;
+    int rc = fstage_enqueue(c, f, c->stream);
+    if (rc == FX_OK && xslots > 0) rc = xstage_enqueue(c, 0, xslots, c->stream);
+    c->capturing = 0;
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    if (rc != FX_OK || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc != FX_OK ? rc : fail(FX_EHIP, "subint graph capture failed: %s", hipGetErrorString(ee));
     }
-    const bool ingest = c->ingest;
-    if (ingest) HIPCHK(hipStreamWaitEvent(c->stream, c->copied[f.slot], 0));
-    TIMED(TM_GRAPH, c->stream, hipGraphLaunch(exec, c->stream));
-    if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], c->stream));
-    if (c->timing) c->nl[2]++;
-    c->last_nffts = f.npad;
+    sg.graph = g;
+    if (args) {
+        size_t n = 0;
+        HIPCHK(hipGraphGetNodes(g, nullptr, &n));
+        std::vector<hipGraphNode_t> nodes(n);
+        HIPCHK(hipGraphGetNodes(g, nodes.data(), &n));
+        for (hipGraphNode_t nd : nodes) {
+            hipGraphNodeType t;
+            if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+            hipKernelNodeParams kp;
+            if (hipGraphKernelNodeGetParams(nd, &kp) != hipSuccess) continue;
+            if (kp.func == delay_rot_kernel_fn() && kp.kernelParams) {
+                sg.dnode = nd;
+                sg.kp = kp;
+            }
+        }
+        if (!sg.dnode) return FX_EUNSUPPORTED;              // the caller records with device indices instead
+    }
+    for (int i = 0; i < kExecCopies; i++) {
+        const hipError_t ie = hipGraphInstantiate(&sg.exec[i], g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) return fail(FX_EHIP, "subint graph instantiation failed: %s", hipGetErrorString(ie));
+        HIPCHK(hipEventCreateWithFlags(&sg.done[i], hipEventDisableTiming));
+    }
     return FX_OK;
+}
+
+static int subint_graph(fx_context* c, FStageCall& f, const int* iteration, int xslots)
+{
+    // contiguous indices as kernel arguments where the runtime lets the
+    // recorded node's arguments be updated; otherwise from device memory
+    bool args = c->graph_args;
+    for (int k = 1; k < f.numffts && args; k++) args = iteration[k] == iteration[0] + k;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (args) {
+            f.it0 = iteration[0];
+            f.contig = 1;
+        } else {
+            int rc = iter_upload(c, iteration, f, c->stream, true);
+            if (rc) return rc;
+        }
+        const long long key[10] = {f.numffts, f.maxoffset, f.slot, f.fft0, xslots, c->tap, c->xsb, c->fplan,
+                                   c->autoscale, args ? 1 : 0};
+        fx_context::SubGraph* sg = nullptr;
+        for (auto& g : c->graphs)
+            if (std::equal(key, key + 10, g.key)) sg = &g;
+        if (!sg) {
+            if (c->graphs.size() >= kMaxSubGraphs) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                for (auto& g : c->graphs) subgraph_free(g);
+                c->graphs.clear();
+            }
+            fx_context::SubGraph ng{};
+            std::copy(key, key + 10, ng.key);
+            const int rc = subgraph_record(c, f, xslots, args, ng);
+            if (rc == FX_EUNSUPPORTED && args) {
+                // no updatable argument list from this runtime: device indices from now on
+                subgraph_free(ng);
+                c->graph_args = 0;
+                args = false;
+                continue;
+            }
+            if (rc) {
+                subgraph_free(ng);
+                return rc;
+            }
+            c->graphs.push_back(ng);
+            sg = &c->graphs.back();
+        }
+        const int i = sg->next;
+        sg->next = (sg->next + 1) % kExecCopies;
+        if (sg->used[i]) HIPCHK(hipEventSynchronize(sg->done[i]));   // this copy's last launch is done
+        if (args) {
+            int it0 = f.it0, org = f.origin;
+            void* a[kDelayNArgs];
+            for (int k = 0; k < kDelayNArgs; k++) a[k] = sg->kp.kernelParams[k];
+            a[kDelayArgIt0] = &it0;
+            a[kDelayArgOrigin] = &org;
+            hipKernelNodeParams kp = sg->kp;
+            kp.kernelParams = a;
+            kp.extra = nullptr;
+            HIPCHK(hipGraphExecKernelNodeSetParams(sg->exec[i], sg->dnode, &kp));
+        }
+        const bool ingest = c->ingest;
+        if (ingest) HIPCHK(hipStreamWaitEvent(c->stream, c->copied[f.slot], 0));
+        TIMED(TM_GRAPH, c->stream, hipGraphLaunch(sg->exec[i], c->stream));
+        HIPCHK(hipEventRecord(sg->done[i], c->stream));
+        sg->used[i] = 1;
+        if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], c->stream));
+        if (c->timing) c->nl[2]++;
+        c->last_nffts = f.npad;
+        return FX_OK;
+    }
+    return fail(FX_EHIP, "subint graph: no recording");
 }
 
 extern "C" int fx_subint(fx_context* c, int slot, int numffts, const int* iteration, int split, int origin, int fft0,
@@ -708,9 +802,7 @@ extern "C" int fx_subint(fx_context* c, int slot, int numffts, const int* iterat
         if (rc == FX_OK && xslots > 0) rc = xstage_enqueue(c, 0, xslots, c->stream);
         return rc;
     }
-    rc = iter_upload(c, iteration, f, c->stream, true);
-    if (rc) return rc;
-    return subint_graph(c, f, xslots);
+    return subint_graph(c, f, iteration, xslots);
 }
 
 extern "C" int fx_process(fx_context* c, int numffts, const int* iteration, int split)

@@ -248,6 +248,8 @@ __device__ void rotator_fill(const FxPlan& p, const FxDelay& d, const float* __r
 // the FP64 delay-model work per entry, and a table of 160 entries (1024
 // channels) takes five full 32-lane rounds instead of three 64-lane rounds
 // with the last one half idle.
+// Arguments 4 (it0) and 9 (origin) are updated in recorded graphs
+// (kDelayArgIt0, kDelayArgOrigin); keep their positions.
 static constexpr int kDelayWaves = 4;
 __global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
     FxPlan p, const double* __restrict__ delays, const double* __restrict__ fileoff, const int* __restrict__ iteration,
@@ -276,6 +278,8 @@ __global__ __launch_bounds__(64 * kDelayWaves) void fx_delay_rot_kernel(
     }
     if (e.valid) rotator_fill(p, e, subchf, stepchf, step_q, rtab + fa * fx_rtab_len(p), lane, 32);
 }
+
+const void* delay_rot_kernel_fn() { return reinterpret_cast<const void*>(&fx_delay_rot_kernel); }
 
 hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const double* fileoff, const int* iteration,
                                  int it0, int numffts, int nffts_pad, int maxoffset, const int* origin_dev, int origin,

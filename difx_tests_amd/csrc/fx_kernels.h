@@ -61,6 +61,11 @@ hipError_t launch_delay_rotators(const FxPlan& p, const double* delays, const do
                                  int it0, int numffts, int nffts_pad, int maxoffset, const int* origin_dev, int origin,
                                  const float* subchf, const float* stepchf, int step_q, FxDelay* dtab, float2* rtab,
                                  uint8_t* valid_slots, hipStream_t s);
+// The delay / rotator kernel's function as graph kernel nodes record it, and
+// the positions of its first-FFT-index and origin arguments among its
+// kDelayNArgs arguments (a recorded graph's launches update those two).
+const void* delay_rot_kernel_fn();
+constexpr int kDelayArgIt0 = 4, kDelayArgOrigin = 9, kDelayNArgs = 16;
 // The F-stage kernel of radix plan `plan` reads its pass-0 fringe steps b + q*r
 // (r < R0) with scalar loads from the tables' second copy of the steps; returns
 // that q, or 0 when the plan reads them from the LDS.
