@@ -106,11 +106,14 @@ def test_graph_replays_against_eager():
         e.close()
 
 
-# 70 antennas: 5 signal blocks, partial last panels; 48 antennas: 3 blocks,
-# one panel pair per channel at tile 4 (the FP16 non-temporal-load kernel)
-@pytest.mark.parametrize("prec,nant", [("f32", 70), ("f16", 70), ("f32v", 70), ("f16", 48)])
-def test_xtile_does_not_change_results(prec, nant):
-    nchan, nf = 256, 4
+# 70 antennas: 5 signal blocks, partial last panels; 48 and 64 antennas: 3 and
+# 4 blocks, one panel pair per channel at tile 4 (the FP16 kernel that stages
+# one panel and keeps two K chunks in flight), over K loops of 1, 3, 4 and 5
+# chunks of 16 FFTs
+@pytest.mark.parametrize("prec,nant,nf", [("f32", 70, 4), ("f16", 70, 4), ("f32v", 70, 4), ("f16", 48, 4),
+                                          ("f16", 48, 48), ("f16", 64, 64), ("f16", 64, 80)])
+def test_xtile_does_not_change_results(prec, nant, nf):
+    nchan = 256
     p, data, delays, fileoff = _subband(nchan, nant, 16e6, nf, 77)
     out = {}
     for sb in (1, 2, 4):
