@@ -22,6 +22,8 @@
 // of the launch.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fx_common.h"
 #include "fx_kernels.h"
 
@@ -140,19 +142,29 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // H16 (FP32, SB = 1, at most 16 signals): the one block is formed as its
 // top-left 16x16 quarter on v_mfma_f32_16x16x4f32, a quarter of the MFMA
 // work of a 32x32 block for the same K.
+// NTL also means one panel pair per channel (every signal in one diagonal
+// super-tile): one panel is staged, each thread issues one branch-free load
+// per chunk, and the loads run two chunks ahead in two register sets (the
+// waits before the LDS stores then cover only the older chunk; a conditional
+// load would make the compiler wait for every load in flight).
 template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false, bool H16 = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
     constexpr int PW = SB * FX_BLK;                      // signals per panel
     // rows staged per group and panel, and panels staged: H16 stages only the
-    // 16 rows its quarter block reads, of its one (diagonal) panel
+    // 16 rows its quarter block reads, of its one (diagonal) panel; NTL the
+    // one (diagonal) panel of its grid
     constexpr int PWL = H16 ? FX_BLK / 2 : PW;
-    constexpr int NPL = H16 ? 1 : 2;
+    constexpr int NPL = (H16 || NTL) ? 1 : 2;
+    constexpr int PF = NTL ? 2 : 1;                      // chunks in flight per thread
+    static_assert(PF == 1 || NSUB == 1, "two chunks in flight: one chunk per step");
     constexpr int NELEM = NPL * PWL * KC;                // 16-byte rows per chunk
     constexpr int NLD = (NSUB * NELEM + 64 * NW - 1) / (64 * NW);
     constexpr bool PART = (NSUB * NELEM) % (64 * NW) != 0;   // threads past the step's rows load nothing
-    __shared__ __align__(16) uint4 lds[2][NSUB][2][KC][PW];   // [stage][chunk][panel][group][signal]
+    // [stage][chunk][panel][group][signal]; H16 keeps two panels' space, which
+    // its partial-sum reduction reuses
+    __shared__ __align__(16) uint4 lds[2][NSUB][NTL ? 1 : 2][KC][PW];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
     // the launch has 64*NW threads: the panel of a load slot is then a
@@ -174,6 +186,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     }
     const int SJ = SI + pr;
     const bool diag = SI == SJ;
+    if constexpr (NTL) __builtin_assume(diag);            // one panel pair per channel
 
     // blocks of this wave.  SB = 4 with eight waves (FP16, FP8; SUBT8): a 1x2
     // sub-tile per wave (row wave/2, columns 2*(wave%2) + j in slot j), so a
@@ -257,10 +270,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     // diagonal workgroup skips its second panel (a workgroup- or wave-uniform
     // branch; that LDS is not read).
     static_assert(H16 || NELEM % (64 * NW) == 0, "whole loads per thread");
-    uint4 st[NLD];
+    uint4 st[PF][NLD];
 #pragma unroll
-    for (int l = 0; l < NLD; l++) st[l] = make_uint4(0, 0, 0, 0);
-    auto gload = [&](int sp) {
+    for (int q = 0; q < PF; q++)
+#pragma unroll
+        for (int l = 0; l < NLD; l++) st[q][l] = make_uint4(0, 0, 0, 0);
+    auto gload = [&](int sp, uint4 (&r)[NLD]) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * NW;
@@ -274,11 +289,15 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             // out back to back (a branch between them serialises them)
             const int chunk = sp * NSUB + sub;
             const int chc = chunk < nchunk ? chunk : nchunk - 1;
-            if (NSUB > 1 || !(diag && pan))
-                st[l] = ld16<NTL>(gbase + (chc * cstride + g * Spad + (sig < Spad ? sig : Spad - 1)));
+            // a wave-uniform chunk base and a 32-bit per-thread byte offset
+            // (the scalar-base load form: no per-load address in registers)
+            const char* cb = reinterpret_cast<const char*>(gbase + chc * cstride);
+            const unsigned boff = (unsigned)(g * Spad + (sig < Spad ? sig : Spad - 1)) * 16u;
+            if (NPL == 1 || NSUB > 1 || !(diag && pan))
+                r[l] = ld16<NTL>(reinterpret_cast<const uint4*>(cb + boff));
         }
     };
-    auto sstore = [&](int stage) {
+    auto sstore = [&](int stage, const uint4 (&r)[NLD]) {
 #pragma unroll
         for (int l = 0; l < NLD; l++) {
             const int e = tid + l * 64 * NW;
@@ -286,7 +305,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             const int sub = e / NELEM, e2 = e - sub * NELEM;
             const int pan = NPL == 1 ? 0 : e2 / (KC * PWL), rem = e2 - pan * (KC * PWL);
             const int g = rem / PWL, sgl = rem - g * PWL;
-            lds[stage][sub][pan][g][sgl] = st[l];
+            lds[stage][sub][pan][g][sgl] = r[l];
         }
     };
 
@@ -300,18 +319,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     f32x4 acr16 = {}, aci16 = {};
     const int row = lane & 31, kg = lane >> 5;
     const int r16 = lane & 15, k4 = lane >> 4;
-    const int pb = diag ? 0 : 1;                          // panel of the B operand
+    const int pb = (NPL == 1 || diag) ? 0 : 1;            // panel of the B operand
 
     if (nstep > 0) {
-        gload(0);
-        sstore(0);
+        gload(0, st[0]);
+        sstore(0, st[0]);
     }
+    // (launches have at least one chunk; past the last one, the last again)
+    if constexpr (PF == 2) gload(1, st[1]);
     __syncthreads();
     // KSPLIT: K step k of the launch belongs to wave k % NW
     auto mine = [&](int k) { return !KSPLIT || (k & (NW - 1)) == wave; };
-    for (int sp = 0; sp < nstep; sp++) {
-        const int stg = sp & 1;
-        if (sp + 1 < nstep) gload(sp + 1);
+    // the MFMAs of the chunks of step sp from LDS stage stg
+    auto compute = [&](int sp, int stg) {
 #pragma unroll
         for (int sub = 0; sub < NSUB; sub++) {
         if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
@@ -435,8 +455,33 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             }
         }
         }   // sub
-        if (sp + 1 < nstep) sstore(stg ^ 1);
-        __syncthreads();
+    };
+    if constexpr (PF == 1) {
+        for (int sp = 0; sp < nstep; sp++) {
+            const int stg = sp & 1;
+            if (sp + 1 < nstep) gload(sp + 1, st[0]);
+            compute(sp, stg);
+            if (sp + 1 < nstep) sstore(stg ^ 1, st[0]);
+            __syncthreads();
+        }
+    } else {
+        // chunk k travels in register set k & 1 and is staged in LDS stage
+        // k & 1: step sp issues chunk sp + 2 into the set chunk sp left,
+        // computes chunk sp, then stages chunk sp + 1.  Loads and stores run
+        // unconditionally (past the last chunk they repeat it, and compute
+        // skips steps past it): with every path issuing the same loads, the
+        // compiler's waits before the stores cover only the older chunk.
+        auto step = [&](int sp, auto par) {
+            constexpr int P = decltype(par)::value;
+            gload(sp + 2, st[P]);
+            compute(sp, P);
+            sstore(P ^ 1, st[P ^ 1]);
+            __syncthreads();
+        };
+        for (int sp = 0; sp < nstep; sp += 2) {
+            step(sp, std::integral_constant<int, 0>{});
+            step(sp + 1, std::integral_constant<int, 1>{});
+        }
     }
 
     if constexpr (KSPLIT) {
