@@ -229,15 +229,14 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     *out = nullptr;
     if (prm->prec < FX_PREC_F32 || prm->prec > FX_PREC_F32V) return fail(FX_EINVAL, "bad precision %d", prm->prec);
     if (prm->max_ffts < 1) return fail(FX_EINVAL, "max_ffts must be >= 1");
+    FxPlan plan;
+    int rc = make_plan(prm, &plan);   // parameters are checked before any device call
+    if (rc) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(FX_EHIP, "no HIP device available");
     fx_context* c = new fx_context();
-    int rc = make_plan(prm, &c->p);
-    if (rc) {
-        delete c;
-        return rc;
-    }
+    c->p = plan;
     FxPlan& p = c->p;
     c->prec = prm->prec;
     c->fprec = prm->prec == FX_PREC_F32V ? FX_PREC_F32 : prm->prec;   // F-stage output format

@@ -196,6 +196,14 @@ void processAntennasAndBaselinePara(int* nant, int* numffts, int* fftchannels, d
     const int nbl = baseline_count(na);
     // bytes read: samples [0, numffts*split*F) of every antenna (kernel.c:730)
     const long long nbytes = (long long)nf * split * nc;
+    if (nf <= 0) {
+        // the reference's FFT loop runs no iteration: accumulators and the
+        // antenna buffers are left as they are, the pointers copied out
+        printf("Time taken station+unpack+fringe+fft+frac+conj+base (one graph launch) in ms : %f \n", 0.0);
+        memcpy(visibilities_out, visibilities, (size_t)(nbl > 0 ? nbl : 0) * sizeof(cf32**));
+        memcpy(baselineCount_out, baselineCount, (size_t)(nbl > 0 ? nbl : 0) * sizeof(int));
+        return;
+    }
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     fx_context* ctx = session_ctx(na, nc, *lo, *bw, nf, nbytes);

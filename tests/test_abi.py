@@ -78,3 +78,59 @@ def test_subband_grammar_round_trip():
         [(256, 8e6, 1.65e9, 40, [0, 2]), (1024, 32e6, 1.7e9, 10, None)]
     again = cfgmod.parse_config_text(cfgmod.format_config(cfg), warn=None)
     assert again == cfg
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(nchan=1000), -1),            # not a power of two
+    (dict(nchan=128), -4),             # FFT length 256 below the supported range
+    (dict(nant=0), -1),
+    (dict(bw=0.0), -1),
+    (dict(lo=-1.0), -1),
+    (dict(lo=2.2e9), -1),              # the reference's (int)lo needs lo < 2^31
+    (dict(prec=7), -1),
+    (dict(max_ffts=0), -1),
+    (dict(iscomplex=2), -1),
+])
+def test_create_rejects_bad_parameters_before_any_device_call(kw, code):
+    # fx_create checks its parameters before it looks for a GPU, so these
+    # fail the same way with or without one
+    import ctypes as C
+    lib = engine.load_library()
+    prm = dict(nant=4, nchan=1024, lo=1.65e9, bw=32e6, prec=0, max_ffts=16, max_bytes_per_ant=0, iscomplex=0)
+    prm.update(kw)
+    p = engine._Params(**prm)
+    ctx = C.c_void_p(1)
+    lib.fx_create.argtypes = [C.c_void_p, C.c_void_p]
+    assert lib.fx_create(C.byref(p), C.byref(ctx)) == code
+    assert not ctx.value
+    lib.fx_last_error.restype = C.c_char_p
+    assert lib.fx_last_error()
+
+
+def test_kernelh_zero_ffts_leaves_accumulators():
+    # numffts = 0: the reference's FFT loop (kernel.c:725) runs no iteration,
+    # so nothing is accumulated; the pointer arrays are still copied out
+    import ctypes as C
+    import numpy as np
+    lib = engine.load_library()
+    nant, nchan = 3, 512
+    nbl = nant * (nant - 1) // 2
+    vis = np.arange(nbl * 4 * nchan * 2, dtype=np.float32).reshape(nbl, 4, nchan * 2)
+    before = vis.copy()
+    P = C.c_void_p
+    rows = [(P * 4)(*[vis[b, q].ctypes.data for q in range(4)]) for b in range(nbl)]
+    visp = (P * nbl)(*[C.cast(r, P) for r in rows])
+    out = (P * nbl)()
+    cnt = (C.c_int * nbl)(5, 6, 7)
+    cnt_out = (C.c_int * nbl)()
+    i = lambda v: C.byref(C.c_int(v))
+    d = lambda v: C.byref(C.c_double(v))
+    null = (P * nant)()
+    lib.processAntennasAndBaselinePara.argtypes = [P] * 22 + [C.c_int]
+    lib.processAntennasAndBaselinePara.restype = None
+    lib.processAntennasAndBaselinePara(
+        i(nant), i(0), i(2 * nchan), (C.c_double * nant)(), d(1 / 64e6), null, null, null, null, null,
+        i(2), i(32), i(16), i(0), d(1.65e9), i(nchan), visp, cnt, (C.c_int * 1)(), d(32e6), out, cnt_out, 1)
+    assert np.array_equal(vis, before)
+    assert list(cnt) == [5, 6, 7] and list(cnt_out) == [5, 6, 7]
+    assert [out[b] for b in range(nbl)] == [visp[b] for b in range(nbl)]
