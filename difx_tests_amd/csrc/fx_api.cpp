@@ -283,7 +283,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     ALLOC(c->d_tw, sizeof(float2) * p.F);
     ALLOC(c->d_subchf, sizeof(float) * p.stride);
     ALLOC(c->d_stepchf, sizeof(float) * p.nfstep);
-    c->xin_bytes = (size_t)p.nchan * c->nfg * p.Spad * xin_row_bytes(c->prec);
+    c->xin_bytes = (size_t)(c->nfg / c->kgroups) * p.nchan * fx_xin_pitch(c->kgroups, p.Spad) * xin_row_bytes(c->prec);
     ALLOC(c->d_xin, c->xin_bytes);
     c->vacc_bytes = (size_t)p.nchan * p.nbp * FX_BLK * FX_BLK * sizeof(float2);
     ALLOC(c->d_vacc, c->vacc_bytes);

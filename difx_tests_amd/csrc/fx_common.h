@@ -4,9 +4,11 @@
 //   packed input   u8   [nant][bytes_per_ant]           2-bit, 4 samples/byte
 //                                                        (kernel.c:195-200 layout)
 //   delay table    FxDelay [nffts_pad][nant]             FP64 per (FFT, antenna)
-//   X input        T    [nchan][nfg][Spad][G]            corner-turned spectra,
-//                        (FP32: G = 2, rows (re0, re1, im0, im1))
-//                        signal s = 2*antenna + pol, G consecutive FFTs per group
+//   X input        16-byte rows [nfg/kc][nchan][kc][Spad]  corner-turned spectra,
+//                        one row = G consecutive FFTs of one (channel, signal)
+//                        (FP32: G = 2, rows (re0, re1, im0, im1)), signal
+//                        s = 2*antenna + pol, kc groups per X-engine K chunk;
+//                        channels fx_xin_pitch rows apart
 //   visibilities   f32x2 [nchan][nbp][32*32]             32x32 blocks of the
 //                        Hermitian signal matrix, upper block triangle I <= J
 //   counts         int  [nant*nant]                      valid-FFT counts per pair
@@ -49,6 +51,19 @@ struct FxPlan {
 static inline __host__ __device__ int fx_rtab_len(const FxPlan& p)
 {
     return p.sub + 2 * p.nstep + p.stride + p.nfstep;
+}
+
+// Rows between consecutive channels of one K chunk of the X input: kc*Spad
+// plus five 128-byte lines.  The F-stage writes one 16-byte row per (channel,
+// signal), and eight signals' rows (eight workgroups) fill a line.  At a
+// power-of-two pitch the rows one workgroup writes fall into few L2 sets, and
+// lines left the L2 before their eight signals met there: the F-stage wrote
+// 1.14x to 3.41x the spectra's bytes over the four shapes measured (64 and
+// 256 antennas, FP16 and FP8 rows), 1.04x to 2.18x with one line of padding
+// and 1.03x to 1.50x with five.
+static inline __host__ __device__ long long fx_xin_pitch(int kc, int Spad)
+{
+    return (long long)kc * Spad + 40;
 }
 
 // Upper-triangle block-pair index of (I, J), I <= J < nblk, row-major.

@@ -846,9 +846,10 @@ void fx_fstage_kernel(FStageArgs args)
     const int ant = (lid >> 1) - grp * p.nant;
     const int fgo = args.fg0 + grp;
     const int sig = 2 * ant + pol;
-    // X input row of (channel 0, this group, this signal); channel c adds c*kc*Spad
+    // X input row of (channel 0, this group, this signal); channel c adds c * fx_xin_pitch rows
     const int kq = fgo / args.kc;
-    const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
+    const long long cpitch = fx_xin_pitch(args.kc, p.Spad);
+    const long long xrow0 = (long long)kq * NCH * cpitch + (long long)(fgo - kq * args.kc) * p.Spad + sig;
     int tid = threadIdx.x;
 
     for (int e = tid; e < NTW1L; e += NT)
@@ -1231,7 +1232,7 @@ void fx_fstage_kernel(FStageArgs args)
         for (int r = 0; r < RO; r++) {
             const int c = out_chan<Gm>(tid, m, r);
             const int ou = m * RO + r;
-            const long long row = xrow0 + (long long)c * args.kc * p.Spad;
+            const long long row = xrow0 + (long long)c * cpitch;
             if constexpr (PREC == FX_PREC_F16) {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 4);
 #pragma unroll
@@ -1322,7 +1323,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
     const int fgo = args.fg0 + grp;
     const int sig = 2 * ant + pol;
     const int kq = fgo / args.kc;
-    const long long xrow0 = (((long long)kq * NCH) * args.kc + (fgo - kq * args.kc)) * p.Spad + sig;
+    const long long cpitch = fx_xin_pitch(args.kc, p.Spad);
+    const long long xrow0 = (long long)kq * NCH * cpitch + (long long)(fgo - kq * args.kc) * p.Spad + sig;
     int tid = threadIdx.x;
 
     for (int e = tid; e < R0 * (R1 + 1); e += NT)
@@ -1348,7 +1350,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
 #pragma unroll
                 for (int r = 0; r < RL; r++) {
                     const int c = tid + m * NT + r * PL;
-                    store_slot<PREC>(args, xrow0 + (long long)c * args.kc * p.Spad, gi,
+                    store_slot<PREC>(args, xrow0 + (long long)c * cpitch, gi,
                                      make_float2(0.0f, 0.0f), 0.0f);
                     if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = make_float2(0.0f, 0.0f);
                 }
@@ -1415,7 +1417,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
                         const float2 y = cmul(fr, X);
                         float si = 1.0f;
                         if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
-                        store_slot<PREC>(args, xrow0 + (long long)c * args.kc * p.Spad, gi, y, si);
+                        store_slot<PREC>(args, xrow0 + (long long)c * cpitch, gi, y, si);
                         if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
                     }
                 }

@@ -21,7 +21,7 @@ struct FStageArgs {
     const float2* tw;             // [F] exp(-2 pi i m / F)
     const float* subchf;          // [stride] f32 channel frequencies (kernel.c:164)
     const float* stepchf;         // [nfstep] (kernel.c:165)
-    void* out;                    // X input, [nfg/kc][nchan][kc][Spad] rows of G FFTs
+    void* out;                    // X input, [nfg/kc][nchan][kc][Spad] rows of G FFTs, channel pitch fx_xin_pitch
     int kc;                       // groups per X-engine K chunk
     int fg0;                      // first group written by this launch
     int ngroups;                  // groups in this launch
@@ -34,7 +34,7 @@ struct FStageArgs {
 
 struct XArgs {
     FxPlan p;
-    const void* xin;              // [nfg/KC][nchan][KC][Spad] rows of G FFTs
+    const void* xin;              // [nfg/KC][nchan][KC][Spad] rows of G FFTs, channel pitch fx_xin_pitch
     int fg0, ngroups;             // group range reduced by this launch (multiples of KC)
     float2* vacc;                 // [nchan][nbp][32*32]
     int accumulate;               // 0: overwrite, 1: add

@@ -139,7 +139,7 @@ extern "C" int fx_process_noalloc(const fx_params* prm, const uint8_t* data, lon
     // per chunk of K FFTs: delay entries, validity bytes, rotator tables, X input
     auto per_chunk = [&](long long K) {
         return up256(sizeof(FxDelay) * K * p.nant) + up256((size_t)K * p.nant) +
-               up256(sizeof(float2) * K * p.nant * fx_rtab_len(p)) + up256((size_t)p.nchan * (K / G) * p.Spad * 16);
+               up256(sizeof(float2) * K * p.nant * fx_rtab_len(p)) + up256((size_t)(K / G / kc) * p.nchan * fx_xin_pitch(kc, p.Spad) * 16);
     };
     long long K = npad;
     while (K > gran && fixed + per_chunk(K) > kScratchBytes) K -= gran;
@@ -172,7 +172,7 @@ extern "C" int fx_process_noalloc(const fx_params* prm, const uint8_t* data, lon
     FxDelay* dtab = reinterpret_cast<FxDelay*>(take(sizeof(FxDelay) * K * p.nant));
     uint8_t* valid = take((size_t)K * p.nant);
     float2* rtab = reinterpret_cast<float2*>(take(sizeof(float2) * K * p.nant * fx_rtab_len(p)));
-    const size_t xin_bytes = (size_t)p.nchan * (K / G) * p.Spad * 16;
+    const size_t xin_bytes = (size_t)(K / G / kc) * p.nchan * fx_xin_pitch(kc, p.Spad) * 16;
     void* xin = take(xin_bytes);
 
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -259,8 +259,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     }
 
     // chunk q of channel c: KC*Spad contiguous rows at ((q*nchan + c)*KC)*Spad
-    const long long cstride = (long long)a.p.nchan * KC * Spad;
-    const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * KC * Spad;
+    const long long cpitch = fx_xin_pitch(KC, Spad), cstride = (long long)a.p.nchan * cpitch;
+    const uint4* gbase = reinterpret_cast<const uint4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * cpitch;
     const int nchunk = a.ngroups / KC;
     const int nstep = (nchunk + NSUB - 1) / NSUB;
     // per-thread loads of step sp: element e = (chunk in step, panel, group,
@@ -574,9 +574,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     const bool diag = SI == SJ;
     const int nq = diag ? SB * (SB + 1) / 2 : SB * SB;
     const int pb = diag ? 0 : 1;
-    const long long cstride = (long long)a.p.nchan * KC * Spad;
-    const float4* gbase =
-        reinterpret_cast<const float4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * KC * Spad;
+    const long long cpitch = fx_xin_pitch(KC, Spad), cstride = (long long)a.p.nchan * cpitch;
+    const float4* gbase = reinterpret_cast<const float4*>(a.xin) + ((long long)(a.fg0 / KC) * a.p.nchan + c) * cpitch;
     const int nchunk = a.ngroups / KC;
     const int n = lane & 31, kg = lane >> 5;
 
