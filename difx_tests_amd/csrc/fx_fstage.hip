@@ -21,6 +21,7 @@
 // polarisation, and keeps the G spectra of every channel it owns in
 // registers, so each (channel, signal) run of G FFTs leaves in one 16-byte
 // store.
+#include <atomic>
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
@@ -779,6 +780,58 @@ int fstage_step_q(int lg, int plan, int iscomplex)
     }
 }
 
+// Work of F-stage workgroup `lid` (numbered after the XCD-aware reordering):
+// tiles of `tile` antennas outermost, then the launch's FFT groups, then the
+// tile's signals.  The eight workgroups whose rows share a 128-byte line of
+// the X input (four antennas, two polarisations; tile is a multiple of four)
+// run side by side, the two polarisations of an antenna (same samples, same
+// rotator tables) are neighbours, and one signal's groups follow each other
+// closely, so its FP8 inverse scales stay in the L2 (a group-major order
+// re-read them from HBM for every group).  The last tile holds nant % tile
+// antennas.
+__device__ __forceinline__ void fstage_work(int lid, int nant, int ngroups, int tile, int& grp, int& ant, int& pol)
+{
+    const int full = nant / tile;
+    const int per_tile = 2 * tile * ngroups;
+    int t, r, w;
+    if (lid < full * per_tile) {
+        t = lid / per_tile;
+        r = lid - t * per_tile;
+        w = 2 * tile;
+    } else {
+        t = full;
+        r = lid - full * per_tile;
+        w = 2 * (nant - tile * full);
+    }
+    grp = r / w;
+    const int sl = r - grp * w;
+    ant = tile * t + (sl >> 1);
+    pol = sl & 1;
+}
+
+// Antennas per work tile: enough that at most eight FFT groups are in flight
+// on an XCD (32 CUs x the kernel's resident workgroups per CU), and at least
+// four.  More groups in flight put more lines 2-8 KB apart (one per group and
+// channel) into the L2 at once, and the F-stage's writes grew from 1.04x to
+// 1.14x the spectra at 256 antennas x 1024 channels FP8 (six workgroups per
+// CU: 16 antennas per tile); at 64 x 4096 (two per CU) four antennas measured
+// fastest.  The occupancy query runs once per kernel, outside stream capture;
+// a launch captured before it uses four.
+static constexpr int kMinTileAnt = 4;
+static int fstage_tile_ant(std::atomic<int>& cache, const void* fn, int nthreads, hipStream_t s)
+{
+    int t = cache.load(std::memory_order_relaxed);
+    if (t > 0) return t;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return kMinTileAnt;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nthreads, 0) != hipSuccess || nb < 1) nb = 1;
+    t = kMinTileAnt;
+    while (32 * nb > 16 * t) t *= 2;
+    cache.store(t, std::memory_order_relaxed);
+    return t;
+}
+
 template <int LG, int V, int PREC, bool TAP, bool CPLX = false>
 __global__ __launch_bounds__((Geo<LG, V>::NT)) __attribute__((amdgpu_waves_per_eu(fstage_wpe<LG, V, PREC, CPLX>())))
 void fx_fstage_kernel(FStageArgs args)
@@ -835,15 +888,14 @@ void fx_fstage_kernel(FStageArgs args)
     const FxPlan& p = args.p;
 
     // XCD-aware order: hardware places block b on XCD b % 8; give each XCD a
-    // contiguous range of (group, antenna, pol) work so the G-FFT runs written
-    // by neighbouring antennas meet in the same L2.
+    // contiguous range of work (fstage_work) so the G-FFT runs written by
+    // neighbouring antennas meet in the same L2.
     const int nblocks = gridDim.x;
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
     const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
     const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-    const int pol = lid & 1;
-    const int grp = (lid >> 1) / p.nant;
-    const int ant = (lid >> 1) - grp * p.nant;
+    int grp, ant, pol;
+    fstage_work(lid, p.nant, args.ngroups, args.tile_ant, grp, ant, pol);
     const int fgo = args.fg0 + grp;
     const int sig = 2 * ant + pol;
     // X input row of (channel 0, this group, this signal); channel c adds c * fx_xin_pitch rows
@@ -1317,9 +1369,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
     const int q8 = nblocks >> 3, r8 = nblocks & 7;
     const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
     const int lid = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-    const int pol = lid & 1;
-    const int grp = (lid >> 1) / p.nant;
-    const int ant = (lid >> 1) - grp * p.nant;
+    int grp, ant, pol;
+    fstage_work(lid, p.nant, args.ngroups, args.tile_ant, grp, ant, pol);
     const int fgo = args.fg0 + grp;
     const int sig = 2 * ant + pol;
     const int kq = fgo / args.kc;
@@ -1430,9 +1481,13 @@ void fx_fstage_eo_kernel(FStageArgs args)
 template <int V, int PREC>
 static void launch_eo_prec(const FStageArgs& a, dim3 grid, bool tap, hipStream_t s)
 {
+    static std::atomic<int> tile{0};
     const dim3 block(Geo<kEoLG - 1, V>::NT);
-    if (tap) fx_fstage_eo_kernel<V, PREC, true><<<grid, block, 0, s>>>(a);
-    else fx_fstage_eo_kernel<V, PREC, false><<<grid, block, 0, s>>>(a);
+    FStageArgs b = a;
+    b.tile_ant = fstage_tile_ant(tile, reinterpret_cast<const void*>(&fx_fstage_eo_kernel<V, PREC, false>),
+                                 (int)block.x, s);
+    if (tap) fx_fstage_eo_kernel<V, PREC, true><<<grid, block, 0, s>>>(b);
+    else fx_fstage_eo_kernel<V, PREC, false><<<grid, block, 0, s>>>(b);
 }
 
 template <int V>
@@ -1584,26 +1639,29 @@ hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int l
 
 bool fstage_supported(int lg) { return lg >= 9 && lg <= 15; }
 
+template <int LG, int V, int PREC, bool CPLX>
+static void launch_lgvp(const FStageArgs& a, dim3 grid, bool tap, hipStream_t s)
+{
+    static std::atomic<int> tile{0};
+    const dim3 block(Geo<LG, V>::NT);
+    FStageArgs b = a;
+    b.tile_ant = fstage_tile_ant(tile, reinterpret_cast<const void*>(&fx_fstage_kernel<LG, V, PREC, false, CPLX>),
+                                 (int)block.x, s);
+    if (tap) fx_fstage_kernel<LG, V, PREC, true, CPLX><<<grid, block, 0, s>>>(b);
+    else fx_fstage_kernel<LG, V, PREC, false, CPLX><<<grid, block, 0, s>>>(b);
+}
+
 template <int LG, int V, bool CPLX>
 static hipError_t launch_lgv(const FStageArgs& a, int prec, hipStream_t s)
 {
     const int nblocks = 2 * a.ngroups * a.p.nant;     // one block per polarisation
     if (nblocks <= 0) return hipSuccess;
-    const dim3 grid(nblocks), block(Geo<LG, V>::NT);
+    const dim3 grid(nblocks);
     const bool tap = a.tap != nullptr || a.ltap != nullptr;
     switch (prec) {
-    case FX_PREC_F32:
-        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F32, true, CPLX><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, V, FX_PREC_F32, false, CPLX><<<grid, block, 0, s>>>(a);
-        break;
-    case FX_PREC_F16:
-        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F16, true, CPLX><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, V, FX_PREC_F16, false, CPLX><<<grid, block, 0, s>>>(a);
-        break;
-    case FX_PREC_F8:
-        if (tap) fx_fstage_kernel<LG, V, FX_PREC_F8, true, CPLX><<<grid, block, 0, s>>>(a);
-        else fx_fstage_kernel<LG, V, FX_PREC_F8, false, CPLX><<<grid, block, 0, s>>>(a);
-        break;
+    case FX_PREC_F32: launch_lgvp<LG, V, FX_PREC_F32, CPLX>(a, grid, tap, s); break;
+    case FX_PREC_F16: launch_lgvp<LG, V, FX_PREC_F16, CPLX>(a, grid, tap, s); break;
+    case FX_PREC_F8: launch_lgvp<LG, V, FX_PREC_F8, CPLX>(a, grid, tap, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

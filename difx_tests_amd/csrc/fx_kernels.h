@@ -30,6 +30,7 @@ struct FStageArgs {
     float* ltap;                  // optional decoded 2-bit levels [nffts][nant][2][F] of the
                                   // valid FFTs (real input; only with tap)
     int plan;                     // radix plan, 0 .. fstage_plans(lg) - 1
+    int tile_ant;                 // antennas per work tile (set by launch_fstage)
 };
 
 struct XArgs {
