@@ -350,8 +350,10 @@ __device__ __forceinline__ void bfly_tw(float2& u, float2& v, int q, bool only_s
 
 // In-register forward DFT of size R <= 32, natural order in and out (radix-2
 // decimation in time, fully unrolled).  With HALF only outputs 0..R/2-1 are
-// formed: the last butterfly layer skips its difference half.
-template <int R, bool HALF = false>
+// formed: the last butterfly layer skips its difference half.  With PRE1 the
+// caller has done the first layer, whose butterflies are (x[a], x[a + R/2])
+// with a unit twiddle: x[a] holds their sum and x[a + R/2] their difference.
+template <int R, bool HALF = false, bool PRE1 = false>
 __device__ __forceinline__ void dft(float2 (&x)[R])
 {
     constexpr int BITS = clog2(R);
@@ -359,7 +361,7 @@ __device__ __forceinline__ void dft(float2 (&x)[R])
 #pragma unroll
     for (int i = 0; i < R; i++) y[cbrev(i, BITS)] = x[i];
 #pragma unroll
-    for (int len = 2; len <= R; len <<= 1) {
+    for (int len = PRE1 ? 4 : 2; len <= R; len <<= 1) {
 #pragma unroll
         for (int i = 0; i < R; i += len) {
 #pragma unroll
@@ -1136,17 +1138,28 @@ void fx_fstage_kernel(FStageArgs args)
                     // whose waits then count LDS operations only
                     asm volatile("" ::"s"(stv[R0 - 1].x));
                 }
+                // u[n] = L[n] * (rsub[s] * rstep[k]): the rotator is one contracted
+                // complex product, and the real level enters the DFT's first
+                // butterfly layer, (u[a], u[a + R0/2]), as the multiplier of
+                // two FMAs per component: one operation per sample fewer than
+                // scaling each rotator by its level
+                float lv[R0];
 #pragma unroll
                 for (int r = 0; r < R0; r++) {
-                    // u[n] = L[n] * rsub[s] * rstep[k]: the real level scales the
-                    // fine rotator first, then one contracted complex product
-                    const float l = lt[2 * bp[r * (T0 / 2)]];
+                    lv[r] = lt[2 * bp[r * (T0 / 2)]];
                     if (TAP && args.ltap)
-                        args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + i + r * T0] = l;
+                        args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + i + r * T0] = lv[r];
                     const float2 st = SPERM ? stv[r] : SSTEP ? load_const(spg + r * Q) : sp[r * Q];
-                    x[r] = cmul(make_float2(l * rs.x, l * rs.y), st);
+                    x[r] = cmul(rs, st);
                 }
-                dft<R0>(x);
+#pragma unroll
+                for (int a = 0; a < R0 / 2; a++) {
+                    const float2 ua = x[a];
+                    const float2 vb = make_float2(lv[a + R0 / 2] * x[a + R0 / 2].x, lv[a + R0 / 2] * x[a + R0 / 2].y);
+                    x[a] = make_float2(__builtin_fmaf(lv[a], ua.x, vb.x), __builtin_fmaf(lv[a], ua.y, vb.y));
+                    x[a + R0 / 2] = make_float2(__builtin_fmaf(lv[a], ua.x, -vb.x), __builtin_fmaf(lv[a], ua.y, -vb.y));
+                }
+                dft<R0, false, true>(x);
                 if constexpr (DIF) {
                     twiddle_pow<R0>(x, args.tw, i);
                     float2* wr = buf + Gm::pad(i);
