@@ -373,6 +373,50 @@ __device__ __forceinline__ void dft(float2 (&x)[R])
     for (int i = 0; i < R; i++) x[i] = y[i];
 }
 
+// Tangent form of a twiddle w = f (1 + i t): (t, f) with f = Re w and
+// t = Im w / f, so that w v = f (v.x - t v.y, v.y + t v.x) and u +- w v is
+// two FMAs for the rotation and one per output component, 6 instructions
+// instead of a complex product and two sums (8); the FMA butterfly of Linzer
+// and Feig (1993) with a twiddle known only at run time.  t is formed with
+// the hardware reciprocal (1 ulp).  The rounding error stays within a few ulp
+// of |v| for any angle (a large t comes with a small f); |Re w| < 2^-30 is
+// taken as 2^-30, which changes w v by at most 2^-30 |v.x|.
+__device__ __forceinline__ float2 lf_form(float2 w)
+{
+    const float f = fabsf(w.x) >= 0x1p-30f ? w.x : (w.x < 0.0f ? -0x1p-30f : 0x1p-30f);
+    return make_float2(w.y * __builtin_amdgcn_rcpf(f), f);
+}
+
+// x[r] *= w(r) for r = 1..R-1, then the DFT, with the twiddles of the first
+// butterfly layer's second operands fused into it: w(r) is a complex twiddle
+// for r < R/2 and the tangent form (lf_form) of one for r >= R/2.  Per pair
+// (x[a], x[a + R/2]) a complex product and 6 instructions instead of two
+// complex products and a butterfly (16 instructions fewer per radix-16 DFT).
+template <int R, class WF>
+__device__ __forceinline__ void dft_twin(float2 (&x)[R], WF w)
+{
+#pragma unroll
+    for (int a = 0; a < R / 2; a++) {
+        const float2 u = a == 0 ? x[0] : cmul(x[a], w(a));
+        const float2 v = x[a + R / 2], tf = w(a + R / 2);
+        const float p = __builtin_fmaf(-tf.x, v.y, v.x), q = __builtin_fmaf(tf.x, v.x, v.y);
+        x[a] = make_float2(__builtin_fmaf(tf.y, p, u.x), __builtin_fmaf(tf.y, q, u.y));
+        x[a + R / 2] = make_float2(__builtin_fmaf(-tf.y, p, u.x), __builtin_fmaf(-tf.y, q, u.y));
+    }
+    dft<R, false, true>(x);
+}
+
+// Entry e of a pass-1 twiddle table (rows of R1 + 1, row k, column r):
+// W^(k r) of the R0 R1 points, read from the F-point table tw; with TAN the
+// columns r >= R1/2 in tangent form, as dft_twin applies them.
+template <int R0, int R1, bool TAN>
+__device__ __forceinline__ float2 tw1_entry(const float2* __restrict__ tw, int e, int F)
+{
+    const int r = e % (R1 + 1);
+    const float2 w = tw[((e / (R1 + 1)) * r) % (R0 * R1) * (F / (R0 * R1))];
+    return TAN && r >= R1 / 2 ? lf_form(w) : w;
+}
+
 // Factors of the twiddles W^(r*e), r < R, W = exp(-2 pi i / F) from the
 // table tw: each twiddle is one product of two loaded factors, W^(4ae) *
 // W^(be) for r = 4a + b, so a radix-R butterfly loads R/4 + 3 table entries.
@@ -610,7 +654,9 @@ int fx_fstage_group_size(int prec)
 // 2 when a 2^LG-point sub-transform reads the table of a 2^(LG+1)-point one.
 // fac: when not null, the thread's twiddle factors of this pass (one per
 // butterfly m), loaded before the FFT loop, in place of the table tw.
-template <class Gm, int R, int P, bool TW_LDS, int TWS = 1>
+// TAN: the LDS table tw1 holds tangent forms in its columns r >= R/2
+// (tw1_entry), applied in the DFT's first butterfly layer (dft_twin).
+template <class Gm, int R, int P, bool TW_LDS, int TWS = 1, bool TAN = false>
 __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const float2* __restrict__ tw, int tid,
                                          const TwFac<R>* fac = nullptr)
 {
@@ -634,14 +680,18 @@ __device__ __forceinline__ void mid_pass(float2* buf, const float2* tw1, const f
             // tw1 is k-major with row stride R + 1: a per-lane base and an
             // immediate offset per r, and 16 lanes hit 16 distinct banks
             const float2* tk = tw1 + k * (R + 1);
+            if constexpr (TAN) {
+                dft_twin<R>(x[m], [&](int r) { return tk[r]; });
+            } else {
 #pragma unroll
-            for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tk[r]);
-        } else if (fac) {
-            twiddle_apply<R>(x[m], fac[m]);
+                for (int r = 1; r < R; r++) x[m][r] = cmul(x[m][r], tk[r]);
+                dft<R>(x[m]);
+            }
         } else {
-            twiddle_pow<R>(x[m], tw, k * (TWS * F / (P * R)));
+            if (fac) twiddle_apply<R>(x[m], fac[m]);
+            else twiddle_pow<R>(x[m], tw, k * (TWS * F / (P * R)));
+            dft<R>(x[m]);
         }
-        dft<R>(x[m]);
     }
     __syncthreads();
 #pragma unroll
@@ -738,6 +788,19 @@ template <int LG, int V, int PREC, bool CPLX>
 __host__ __device__ constexpr int fstage_wpe()
 {
     return (LG == 12 && !CPLX && PREC != FX_PREC_F8) ? 4 : Geo<LG, V, CPLX>::WPE;
+}
+
+// Tangent-form twiddles (lf_form) in the first two passes: fewer VALU
+// operations per point, but more live registers.  Below F = 8192 every plan
+// that uses them (Stockham: pass 1 from the LDS table) has room for them and
+// gains 1-2.5% of its VALU instructions; at F >= 8192 the kernels are compiled
+// for the 128-register budget of four waves per SIMD, where the forms brought
+// scratch spills and the 8192-point FP16 F-stage measured 3-4% slower; the
+// 4096-point FP8 kernel of plan 0 also spills with them.
+template <int LG, int V, int PREC, bool CPLX>
+__host__ __device__ constexpr bool fstage_tangent()
+{
+    return LG <= 12 && FPlan<LG, V>::KIND != 1 && !(LG == 12 && V == 0 && PREC == FX_PREC_F8);
 }
 
 // Pass 0 of real input reads its fringe steps with scalar loads when the
@@ -906,8 +969,8 @@ void fx_fstage_kernel(FStageArgs args)
     const long long xrow0 = (long long)kq * NCH * cpitch + (long long)(fgo - kq * args.kc) * p.Spad + sig;
     int tid = threadIdx.x;
 
-    for (int e = tid; e < NTW1L; e += NT)
-        tw1[e] = args.tw[((e / (R1 + 1)) * (e % (R1 + 1))) % NTW1 * (F / NTW1)];
+    constexpr bool TAN = fstage_tangent<LG, V, PREC, CPLX>();
+    for (int e = tid; e < NTW1L; e += NT) tw1[e] = tw1_entry<R0, R1, TAN>(args.tw, e, F);
     if constexpr (!CPLX) {
         if (tid < 32) {
             // code pair e = slot-0 code | slot-1 code << 2; levels of kernel.c:191-192
@@ -1138,26 +1201,60 @@ void fx_fstage_kernel(FStageArgs args)
                     // whose waits then count LDS operations only
                     asm volatile("" ::"s"(stv[R0 - 1].x));
                 }
-                // u[n] = L[n] * (rsub[s] * rstep[k]): the rotator is one contracted
-                // complex product, and the real level enters the DFT's first
-                // butterfly layer, (u[a], u[a + R0/2]), as the multiplier of
-                // two FMAs per component: one operation per sample fewer than
-                // scaling each rotator by its level
-                float lv[R0];
+                if constexpr (TAN) {
+                    // u[n] = L[n] * rsub[s] * rstep[k].  The fine rotator is the
+                    // same for the thread's R0 samples (T0 is a multiple of SUB),
+                    // so it is split as rsub[s] = f (1 + i t) (tangent form): f
+                    // scales the levels, the step rotators enter the DFT's first
+                    // butterfly layer, (u[a], u[a + R0/2]), with the scaled levels
+                    // as the multipliers of two FMAs per component, and (1 + i t)
+                    // rotates the layer's outputs with two FMAs each: six
+                    // operations per sample where a complex product per rotator
+                    // and the level layer took seven
+                    const float2 rtf = lf_form(rs);
+                    float lv[R0];
+                    float2 st[R0];
 #pragma unroll
-                for (int r = 0; r < R0; r++) {
-                    lv[r] = lt[2 * bp[r * (T0 / 2)]];
-                    if (TAP && args.ltap)
-                        args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + i + r * T0] = lv[r];
-                    const float2 st = SPERM ? stv[r] : SSTEP ? load_const(spg + r * Q) : sp[r * Q];
-                    x[r] = cmul(rs, st);
-                }
+                    for (int r = 0; r < R0; r++) {
+                        const float l = lt[2 * bp[r * (T0 / 2)]];
+                        if (TAP && args.ltap)
+                            args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + i + r * T0] = l;
+                        lv[r] = l * rtf.y;
+                        st[r] = SPERM ? stv[r] : SSTEP ? load_const(spg + r * Q) : sp[r * Q];
+                    }
 #pragma unroll
-                for (int a = 0; a < R0 / 2; a++) {
-                    const float2 ua = x[a];
-                    const float2 vb = make_float2(lv[a + R0 / 2] * x[a + R0 / 2].x, lv[a + R0 / 2] * x[a + R0 / 2].y);
-                    x[a] = make_float2(__builtin_fmaf(lv[a], ua.x, vb.x), __builtin_fmaf(lv[a], ua.y, vb.y));
-                    x[a + R0 / 2] = make_float2(__builtin_fmaf(lv[a], ua.x, -vb.x), __builtin_fmaf(lv[a], ua.y, -vb.y));
+                    for (int a = 0; a < R0 / 2; a++) {
+                        const float2 sa = st[a], sb = st[a + R0 / 2];
+                        const float mx = lv[a + R0 / 2] * sb.x, my = lv[a + R0 / 2] * sb.y;
+                        const float la = lv[a], t = rtf.x;
+                        const float2 u0 = make_float2(__builtin_fmaf(la, sa.x, mx), __builtin_fmaf(la, sa.y, my));
+                        const float2 u1 = make_float2(__builtin_fmaf(la, sa.x, -mx), __builtin_fmaf(la, sa.y, -my));
+                        x[a] = make_float2(__builtin_fmaf(-t, u0.y, u0.x), __builtin_fmaf(t, u0.x, u0.y));
+                        x[a + R0 / 2] = make_float2(__builtin_fmaf(-t, u1.y, u1.x), __builtin_fmaf(t, u1.x, u1.y));
+                    }
+                } else {
+                    // u[n] = L[n] * (rsub[s] * rstep[k]): the rotator is one contracted
+                    // complex product, and the real level enters the DFT's first
+                    // butterfly layer, (u[a], u[a + R0/2]), as the multiplier of
+                    // two FMAs per component: one operation per sample fewer than
+                    // scaling each rotator by its level
+                    float lv[R0];
+#pragma unroll
+                    for (int r = 0; r < R0; r++) {
+                        lv[r] = lt[2 * bp[r * (T0 / 2)]];
+                        if (TAP && args.ltap)
+                            args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + i + r * T0] = lv[r];
+                        const float2 st = SPERM ? stv[r] : SSTEP ? load_const(spg + r * Q) : sp[r * Q];
+                        x[r] = cmul(rs, st);
+                    }
+#pragma unroll
+                    for (int a = 0; a < R0 / 2; a++) {
+                        const float2 ua = x[a], xb = x[a + R0 / 2];
+                        const float la = lv[a], lb = lv[a + R0 / 2];
+                        const float2 vb = make_float2(lb * xb.x, lb * xb.y);
+                        x[a] = make_float2(__builtin_fmaf(la, ua.x, vb.x), __builtin_fmaf(la, ua.y, vb.y));
+                        x[a + R0 / 2] = make_float2(__builtin_fmaf(la, ua.x, -vb.x), __builtin_fmaf(la, ua.y, -vb.y));
+                    }
                 }
                 dft<R0, false, true>(x);
                 if constexpr (DIF) {
@@ -1176,7 +1273,7 @@ void fx_fstage_kernel(FStageArgs args)
         __syncthreads();
 
         if constexpr (XL) {
-            mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
+            mid_pass<Gm, R1, R0, true, 1, TAN>(buf, tw1, args.tw, tid);
             // third pass (radix 16 on sub-transforms of 256), butterflies k
             // and 256 + k in the two half-waves; its outputs stay in registers
             const int up = (tid >> 5) & 1;
@@ -1236,7 +1333,7 @@ void fx_fstage_kernel(FStageArgs args)
             dif_pass<Gm, R1, F / R0>(buf, args.tw, tid);
             if constexpr (Gm::NP == 4) dif_pass<Gm, R2, F / (R0 * R1)>(buf, args.tw, tid);
         } else {
-            mid_pass<Gm, R1, R0, true>(buf, tw1, args.tw, tid);
+            mid_pass<Gm, R1, R0, true, 1, TAN>(buf, tw1, args.tw, tid);
             if constexpr (Gm::NP == 4)
                 mid_pass<Gm, R2, R0 * R1, false>(buf, tw1, args.tw, tid, TWREG ? fac2 : nullptr);
         }
@@ -1391,8 +1488,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
     const long long xrow0 = (long long)kq * NCH * cpitch + (long long)(fgo - kq * args.kc) * p.Spad + sig;
     int tid = threadIdx.x;
 
-    for (int e = tid; e < R0 * (R1 + 1); e += NT)
-        tw1[e] = args.tw[((e / (R1 + 1)) * (e % (R1 + 1))) % NTW1 * (F / NTW1)];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT) tw1[e] = tw1_entry<R0, R1, false>(args.tw, e, F);
     for (int e = tid; e < 256; e += NT) {
         const float hi = 3.3359f;
         const float lv[4] = {-hi, -1.0f, 1.0f, hi};
@@ -1534,8 +1630,7 @@ __global__ __launch_bounds__(Geo<kEoLG - 1>::NT) void fx_fft_c2c_eo_kernel(const
     const float2* x_in = in + (long long)blockIdx.x * F;
     float2* x_out = out + (long long)blockIdx.x * F;
     int tid = threadIdx.x;
-    for (int e = tid; e < R0 * (R1 + 1); e += NT)
-        tw1[e] = tw[((e / (R1 + 1)) * (e % (R1 + 1))) % (R0 * R1) * (F / (R0 * R1))];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT) tw1[e] = tw1_entry<R0, R1, false>(tw, e, F);
     float2 ev[PERL][RL];
 #pragma unroll 1
     for (int h = 0; h < 2; h++) {
@@ -1597,8 +1692,7 @@ __global__ __launch_bounds__(Geo<LG>::NT) void fx_fft_c2c_kernel(const float2* _
     const float2* x_in = in + (long long)blockIdx.x * F;
     float2* x_out = out + (long long)blockIdx.x * F;
     const int tid = threadIdx.x;
-    for (int e = tid; e < R0 * (R1 + 1); e += NT)
-        tw1[e] = tw[((e / (R1 + 1)) * (e % (R1 + 1))) % (R0 * R1) * (F / (R0 * R1))];
+    for (int e = tid; e < R0 * (R1 + 1); e += NT) tw1[e] = tw1_entry<R0, R1, false>(tw, e, F);
 #pragma unroll
     for (int m = 0; m < PER0; m++) {
         const int i = tid + m * NT;
