@@ -82,7 +82,11 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
 
 
-MAX_XBATCH = 16
+# Subints per X-engine launch the autotuner may choose from (its FFT capacity).
+# Each X-engine launch reads and writes the accumulators once, and the
+# F-stages right after it run slower (the clock recovers from the MFMA burst
+# over a few of them), so fewer, deeper launches raise the step rate.
+MAX_XBATCH = 64
 NBUF = 16           # subints of distinct input held in HBM and cycled through
 
 
