@@ -5,21 +5,23 @@ Workload (default): 64 antennas x 2 polarisations, 2-bit real samples at
 128 Msps, 4096 channels (8192-point FFT), 10 ms subintegrations of 156 FFTs,
 FP16 spectra into the MFMA X-engine.  One step = the full hot path over one
 batch of `--xbatch` subintegrations already resident in HBM: FP64 delay
-table, fused unpack/fringe/FFT/fractional-delay F-stage (one launch per
-subint), valid-FFT counts, and one X-engine launch accumulating the batch
-into the visibilities.  The data are synthetic (seeded 2-bit Gaussian noise
-with a 0.1-correlated common signal): 16 consecutive subints of distinct
-samples (658 MB at the default workload, more than the 256 MB Infinity
-Cache), cycled through with advancing global FFT indices, so every subint of
-a step streams fresh input from HBM.
+table and fused unpack/fringe/FFT/fractional-delay F-stage (one launch of
+each per `--fbatch` consecutive subints, 16 by default, the FFT indices of
+the subints one contiguous run), valid-FFT counts, and one X-engine launch
+accumulating the batch into the visibilities.  The data are synthetic
+(seeded 2-bit Gaussian noise with a 0.1-correlated common signal): 16
+consecutive subints of distinct samples (658 MB at the default workload,
+more than the 256 MB Infinity Cache), cycled through with advancing global
+FFT indices, so every subint of a step streams fresh input from HBM.
 
 Prints one JSON line.  `value` is input Gsamples/s (antennas x pols x
 samples processed per second) with plain launches timed by HIP events on the
 engine's stream.  The same line carries the X-engine TFLOPS, the same steps
-as one graph launch per subint, the plain FP32 GPU pipeline on the same
-workload, the roofline of the dominant kernel (HIP-event timings inside the
-timed region) and a bounded CPU baseline (the oracle's FP32 restatement of
-the reference path) on all usable host threads and on one.
+with one delay and one F-stage launch per subint, the same steps as graph
+launches, the plain FP32 GPU pipeline on the same workload, the roofline of
+the dominant kernel (HIP-event timings inside the timed region) and a
+bounded CPU baseline (the oracle's FP32 restatement of the reference path) on
+all usable host threads and on one.
 """
 from __future__ import annotations
 
@@ -52,6 +54,8 @@ def parse():
     ap.add_argument("--workload", default="c3-64ant-4096ch", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--prec", default=None, choices=["f32", "f16", "f8", "f32v"])
     ap.add_argument("--xbatch", type=int, default=0, help="subints per X-engine launch (0: the autotuner's choice)")
+    ap.add_argument("--fbatch", type=int, default=0,
+                    help="subints per delay / F-stage launch (0: 16, or fewer where the X batch needs it)")
     ap.add_argument("--no-autotune", action="store_true", help="default radix plan and X-engine tile, xbatch 8")
     ap.add_argument("--plan", type=int, default=None, help="F-stage radix plan (skips the autotuner)")
     ap.add_argument("--xtile", type=int, default=None, help="X-engine tile 1, 2 or 4 (skips the autotuner)")
@@ -59,6 +63,8 @@ def parse():
     ap.add_argument("--no-subbands", action="store_true", help="skip the eight-subband batch measurement")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-resident (PCIe-inclusive) measurement")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-launch measurements")
+    ap.add_argument("--no-per-subint", action="store_true",
+                    help="skip the measurement with one delay and one F-stage launch per subint")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="target CPU baseline work (s), split between the all-threads and one-thread legs; 0 disables")
     ap.add_argument("--seed", type=int, default=20261205)
@@ -90,16 +96,32 @@ MAX_XBATCH = 64
 NBUF = 16           # subints of distinct input held in HBM and cycled through
 
 
+def fstage_batch(xbatch, want=0):
+    """Subints per delay / F-stage launch: `want`, or the largest power of two
+    up to NBUF that divides the X batch (and NBUF)."""
+    if want:
+        if xbatch % want or NBUF % want:
+            raise SystemExit(f"--fbatch {want} must divide the X batch ({xbatch}) and {NBUF}")
+        return want
+    fb = 1
+    while fb * 2 <= NBUF and xbatch % (fb * 2) == 0:
+        fb *= 2
+    return fb
+
+
 def run_pipeline(w, prec, data, delays, fileoff, nf, steps, warmup, xbatch, tune=True, plan=None, xtile=None,
-                 graph=False, timing=True):
+                 graph=False, timing=True, fbatch=0):
     """Returns (seconds per step, kernel ms per step dict, launch counts, slots
-    per subint, tuning dict).  The input buffer holds NBUF consecutive subints
-    of distinct data; subint u of the run reads subint u mod NBUF of it (global
-    FFT indices (u mod NBUF)*nf + k, buffer offsets of the whole buffer as the
-    reference's split does), so a step streams fresh samples.  graph: every
-    subint is one recorded graph launch (fx_subint) with the X-engine in the
-    step's last one; otherwise plain launches with HIP-event kernel timing.
-    xbatch 0 takes the autotuner's choice."""
+    per F-stage launch, tuning dict).  The input buffer holds NBUF consecutive
+    subints of distinct data; subint u of the run reads subint u mod NBUF of it
+    (global FFT indices (u mod NBUF)*nf + k, buffer offsets of the whole
+    buffer as the reference's split does), so a step streams fresh samples.
+    One delay / F-stage launch covers fbatch consecutive subints of the
+    buffer, whose FFT indices form one contiguous run (fbatch 1: one launch
+    per subint, each subint's slots padded to the F-stage granule).  graph:
+    every F-stage launch is one recorded graph launch (fx_subint) with the
+    X-engine in the step's last one; otherwise plain launches with HIP-event
+    kernel timing.  xbatch 0 takes the autotuner's choice."""
     cap = (xbatch or MAX_XBATCH) * (nf + 64)
     e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec=prec, max_ffts=cap, max_bytes_per_ant=data.shape[1])
     try:
@@ -114,21 +136,25 @@ def run_pipeline(w, prec, data, delays, fileoff, nf, steps, warmup, xbatch, tune
         if not xbatch:
             xbatch = tuned.get("xbatch", 8)
         tuned["xbatch_used"] = xbatch
+        fb = fstage_batch(xbatch, fbatch)
+        tuned["fbatch_used"] = fb
         gran = e.granule
-        slots = (nf + gran - 1) // gran * gran
-        its = [(r * nf + np.arange(nf)).astype(np.int32) for r in range(NBUF)]
+        slots = (fb * nf + gran - 1) // gran * gran
+        nblk = NBUF // fb                          # blocks of fb subints in the buffer
+        its = [(r * fb * nf + np.arange(fb * nf)).astype(np.int32) for r in range(nblk)]
+        nlaunch = xbatch // fb
         u = [0]
 
         def step():
-            for s in range(xbatch):
-                it = its[u[0] % NBUF]
+            for s in range(nlaunch):
+                it = its[u[0] % nblk]
                 u[0] += 1
                 if graph:
-                    e.subint(it, split=NBUF, fft0=s * slots, xslots=xbatch * slots if s == xbatch - 1 else 0)
+                    e.subint(it, split=nblk, fft0=s * slots, xslots=nlaunch * slots if s == nlaunch - 1 else 0)
                 else:
-                    e.fstage(it, NBUF, s * slots)
+                    e.fstage(it, nblk, s * slots)
             if not graph:
-                e.xstage(0, xbatch * slots)
+                e.xstage(0, nlaunch * slots)
 
         e.reset()
         for _ in range(warmup):
@@ -253,16 +279,17 @@ def streaming_ingest(w, prec, data, delays, fileoff, nf, tuned, xbatch, nsub):
     return out
 
 
-def measured_traffic(workload, prec, xbatch, kernel):
+def measured_traffic(workload, prec, xbatch, fbatch, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (copy-kernel calibrated), or None.
-    An F-stage launch covers one subint whatever the number of subints per
-    X-engine launch, so for it a summary taken at any xbatch serves."""
+    An F-stage launch covers fbatch subints whatever the number of subints per
+    X-engine launch, so for it a summary taken at any xbatch and the same
+    fbatch serves."""
     pdir = os.path.join(ROOT, "profiles")
-    paths = [os.path.join(pdir, f"traffic_{workload}_{prec}_xbatch{xbatch}.md")]
+    paths = [os.path.join(pdir, f"traffic_{workload}_{prec}_xbatch{xbatch}_fbatch{fbatch}.md")]
     if kernel == "fx_fstage_kernel" and os.path.isdir(pdir):
         paths += sorted(os.path.join(pdir, n) for n in os.listdir(pdir)
-                        if n.startswith(f"traffic_{workload}_{prec}_xbatch") and n.endswith(".md"))
+                        if n.startswith(f"traffic_{workload}_{prec}_xbatch") and n.endswith(f"_fbatch{fbatch}.md"))
     for path in paths:
         if not os.path.exists(path):
             continue
@@ -274,10 +301,10 @@ def measured_traffic(workload, prec, xbatch, kernel):
     return None, None
 
 
-def measured_valu(workload, prec, kernel):
-    """VALU instructions (wave64) per launch of `kernel` from the committed
-    rocprofv3 SQ_INSTS_VALU summary of this workload, or None."""
-    path = os.path.join(ROOT, "profiles", f"sq_counters_{workload}_{prec}.md")
+def measured_valu(workload, prec, fbatch, kernel):
+    """VALU instructions (wave64) per launch of `kernel` (fbatch subints) from
+    the committed rocprofv3 SQ_INSTS_VALU summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_{workload}_{prec}_fbatch{fbatch}.md")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -347,10 +374,11 @@ def graph_vs_launch(steps, warmup, seed):
     data, delays, fileoff = workloads.make_stream_subints(w, seed, NBUF)
     nf = w.nffts
     out = {"workload": w.name, "subints_per_step": None}
-    sec_l, _, _, _, tuned = run_pipeline(w, "f32", data, delays, fileoff, nf, steps, warmup, 0, True, timing=False)
+    sec_l, _, _, _, tuned = run_pipeline(w, "f32", data, delays, fileoff, nf, steps, warmup, 0, True, timing=False,
+                                         fbatch=1)
     xb = tuned["xbatch_used"]
     sec_g, _, _, _, _ = run_pipeline(w, "f32", data, delays, fileoff, nf, steps, warmup, xb, False,
-                                     tuned["plan_used"], tuned["xtile_used"], graph=True, timing=False)
+                                     tuned["plan_used"], tuned["xtile_used"], graph=True, timing=False, fbatch=1)
     out["subints_per_step"] = xb
     sps = xb * w.samples_per_subint
     out["launch"] = {"ms_per_step": round(sec_l * 1e3, 4), "gsamples_s": round(sps / sec_l / 1e9, 3)}
@@ -375,25 +403,25 @@ def main():
     tune = not (a.no_autotune or a.plan is not None or a.xtile is not None)
     xb_req = a.xbatch or (0 if tune else 8)
     sec, kms, nl, slots, tuned = run_pipeline(w, prec, data, delays, fileoff, nf, a.steps, a.warmup, xb_req, tune,
-                                              a.plan, a.xtile)
-    xbatch = tuned["xbatch_used"]
+                                              a.plan, a.xtile, fbatch=a.fbatch)
+    xbatch, fbatch = tuned["xbatch_used"], tuned["fbatch_used"]
     samples_per_step = xbatch * w.samples_per_subint
     value = samples_per_step / sec / 1e9
 
-    # algorithmic work per launch
-    f_launch_bytes = w.samples_per_subint * (0.25 + OUT_BYTES_PER_SAMPLE[prec])
+    # algorithmic work per launch (an F-stage launch covers fbatch subints)
+    f_launch_bytes = fbatch * w.samples_per_subint * (0.25 + OUT_BYTES_PER_SAMPLE[prec])
     x_flops = 8.0 * 4 * w.nbl_with_autos * w.nchan * nf * xbatch
-    f_ms = kms["fstage"] / xbatch            # per F launch
+    f_ms = kms["fstage"] * fbatch / xbatch   # per F launch
     x_ms = kms["xstage"]                      # one X launch per step
     f_gbs = f_launch_bytes / (f_ms * 1e-3) / 1e9
     x_tflops = x_flops / (x_ms * 1e-3) / 1e12
     if kms["fstage"] >= kms["xstage"]:
-        traffic, src = measured_traffic(w.name, prec, xbatch, "fx_fstage_kernel")
+        traffic, src = measured_traffic(w.name, prec, xbatch, fbatch, "fx_fstage_kernel")
         roof = {"kernel": "fx_fstage_kernel", "bound": "hbm", "achieved": round(f_gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": int(f_launch_bytes), "avg_launch_ms": round(f_ms, 4)}
     else:
-        traffic, src = measured_traffic(w.name, prec, xbatch, "fx_xengine_kernel")
+        traffic, src = measured_traffic(w.name, prec, xbatch, fbatch, "fx_xengine_kernel")
         pk = MFMA_PEAK_TFLOPS[prec]
         roof = {"kernel": "fx_xengine_kernel", "bound": "mfma", "achieved": round(x_tflops, 2), "peak": pk,
                 "unit": "TFLOP/s", "frac": round(x_tflops / pk, 4), "traffic": traffic,
@@ -403,7 +431,7 @@ def main():
     # the F-stage's real limiter: VALU issue (non-packed FP32), its profiled
     # instruction count over the live launch time against the issue peak
     valu_roof = None
-    ninst, vsrc = measured_valu(w.name, prec, "fx_fstage_kernel")
+    ninst, vsrc = measured_valu(w.name, prec, fbatch, "fx_fstage_kernel")
     if ninst:
         g = ninst / (f_ms * 1e-3) / 1e9
         valu_roof = {"kernel": "fx_fstage_kernel", "bound": "valu", "achieved": round(g, 1),
@@ -425,6 +453,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "antennas": w.nant, "pols": 2, "nchan": w.nchan, "fs_msps": w.fs / 1e6,
                    "subint_ms": w.subint_s * 1e3, "ffts_per_subint": nf, "subints_per_step": xbatch,
+                   "subints_per_fstage_launch": fbatch,
                    "xengine_prec": prec, "parallelism": "single GPU",
                    "input_in_hbm_bytes": int(data.nbytes), "distinct_subints_cycled": NBUF},
         "xengine_tflops": round(x_tflops, 2),
@@ -439,17 +468,30 @@ def main():
         # the plain FP32 GPU pipeline (vector-ALU X-engine) and the FP32 MFMA one
         for key, p32 in (("fp32_pipeline", "f32v"), ("fp32_mfma_pipeline", "f32")):
             s32, k32, _, _, t32 = run_pipeline(w, p32, data, delays, fileoff, nf, max(3, a.steps // 2), 1, xbatch,
-                                               tune)
+                                               tune, fbatch=fbatch)
             v32 = samples_per_step / s32 / 1e9
             out[key] = {"xengine": p32, "value": round(v32, 3), "ms_per_step": round(s32 * 1e3, 4),
                         "kernel_ms_per_step": {k: round(v, 4) for k, v in k32.items()},
                         "tuning": t32, "speedup_vs_this": round(value / v32, 3)}
+    if fbatch > 1 and not a.no_per_subint:
+        # the same steps with one delay and one F-stage launch per subint
+        s1, k1, _, _, _ = run_pipeline(w, prec, data, delays, fileoff, nf, a.steps, a.warmup, xbatch, False,
+                                       tuned["plan_used"], tuned["xtile_used"], fbatch=1)
+        out["per_subint_launches"] = {"value": round(samples_per_step / s1 / 1e9, 3),
+                                      "ms_per_step": round(s1 * 1e3, 4),
+                                      "kernel_ms_per_step": {k: round(v, 4) for k, v in k1.items()},
+                                      "batched_speedup": round(s1 / sec, 3)}
     if not a.no_graph:
-        # the same steps with every subint as one recorded graph launch
-        sg, _, _, _, _ = run_pipeline(w, prec, data, delays, fileoff, nf, a.steps, a.warmup, xbatch, False,
-                                      tuned["plan_used"], tuned["xtile_used"], graph=True, timing=False)
-        out["graph_launch"] = {"value": round(samples_per_step / sg / 1e9, 3), "ms_per_step": round(sg * 1e3, 4),
-                               "launches_per_subint": 1, "vs_plain_launches": round(sec / sg, 3)}
+        # the same steps with every F-stage launch (and its delay kernel) as
+        # one recorded graph launch, one per subint and one per fbatch subints
+        for key, fbg in (("graph_launch", 1), ("graph_launch_batched", fbatch)):
+            if key == "graph_launch_batched" and fbatch == 1:
+                continue
+            sg, _, _, _, _ = run_pipeline(w, prec, data, delays, fileoff, nf, a.steps, a.warmup, xbatch, False,
+                                          tuned["plan_used"], tuned["xtile_used"], graph=True, timing=False,
+                                          fbatch=fbg)
+            out[key] = {"value": round(samples_per_step / sg / 1e9, 3), "ms_per_step": round(sg * 1e3, 4),
+                        "subints_per_graph_launch": fbg, "vs_plain_launches": round(sec / sg, 3)}
         out["c1_graph_vs_launch"] = graph_vs_launch(max(10, a.steps), a.warmup, a.seed + 1)
     if not a.no_stream:
         sub = np.ascontiguousarray(data[:, : (nf + 1) * w.F // 2])
