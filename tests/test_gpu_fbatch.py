@@ -126,3 +126,43 @@ def test_batched_subint_graph_equals_plain_launches(prec):
     finally:
         for e in engines:
             e.close()
+
+
+def test_batched_fstage_at_the_benchmark_shape():
+    """64 antennas x 4096 channels, FP16, radix plan 2 and X tile 4: one
+    F-stage call over 16 subints of 156 FFTs against one call per subint,
+    both into one X-engine launch (random 2-bit input, delays within the
+    buffer's edge windows)."""
+    from difx_tests_amd import workloads
+    w = workloads.C3
+    nf, nsub = w.nffts, 16
+    delays, fileoff = workloads.delay_model(w, 5)
+    data = synth.random_bytes(w.nant, (nf * nsub + 1) * w.F // 2, 11)
+    res = {}
+    for mode in ("per_subint", "batched"):
+        e = FxEngine(w.nant, w.nchan, w.lo, w.bw, prec="f16", max_ffts=nsub * (nf + 64),
+                     max_bytes_per_ant=data.shape[1])
+        try:
+            e.set_model(delays, fileoff)
+            e.upload(data)
+            e.set_plan(2)
+            e.set_xtile(4)
+            e.reset()
+            g = e.granule
+            if mode == "per_subint":
+                slots = (nf + g - 1) // g * g
+                for s in range(nsub):
+                    e.fstage((s * nf + np.arange(nf)).astype(np.int32), nsub, s * slots)
+                ntot = nsub * slots
+            else:
+                ntot = (nsub * nf + g - 1) // g * g
+                e.fstage(np.arange(nsub * nf, dtype=np.int32), 1, 0)
+            e.xstage(0, ntot)
+            e.sync()
+            res[mode] = e.read_vis()
+        finally:
+            e.close()
+    (ca, cca, aa, aca), (cb, ccb, ab, acb) = res["per_subint"], res["batched"]
+    assert np.array_equal(cca, ccb) and np.array_equal(aca, acb)
+    assert cca.min() > 0
+    assert _rel(cb, ca) <= 1e-6 and _rel(ab, aa) <= 1e-6
