@@ -572,12 +572,15 @@ __device__ __forceinline__ int2 load_delay_ov(const FxDelay* d)
 #endif
 }
 
-// Per-thread storage of the G spectra of the channels a thread owns.
-template <int PREC, int G, int NOUT> struct OutRegs;
+// Per-thread storage of the G spectra of the channels a thread owns.  HALF
+// (FP8 only): the chain holds G/2 FFTs, and each half of a row is stored
+// when its G/2 FFTs are done.
+template <int PREC, int G, int NOUT, bool HALF = false> struct OutRegs;
 // The G FFTs of a group pass through a register shift chain: after each FFT
 // the chain moves down one slot and the new spectrum enters slot G-1, so the
 // FFT loop can stay rolled while every register index is static.
-template <int G, int NOUT> struct OutRegs<FX_PREC_F32, G, NOUT> {
+template <int G, int NOUT, bool HALF> struct OutRegs<FX_PREC_F32, G, NOUT, HALF> {
+    static_assert(!HALF, "half rows: FP8 only");
     float2 v[G][NOUT];
     __device__ __forceinline__ void shift(int) {
 #pragma unroll
@@ -591,7 +594,8 @@ template <int G, int NOUT> struct OutRegs<FX_PREC_F32, G, NOUT> {
     }
     __device__ __forceinline__ void put_last(int u, float2 x, float, int) { v[G - 1][u] = x; }
 };
-template <int G, int NOUT> struct OutRegs<FX_PREC_F16, G, NOUT> {
+template <int G, int NOUT, bool HALF> struct OutRegs<FX_PREC_F16, G, NOUT, HALF> {
+    static_assert(!HALF, "half rows: FP8 only");
     uint32_t v[G][NOUT];
     __device__ __forceinline__ void shift(int) {
 #pragma unroll
@@ -612,26 +616,28 @@ template <int G, int NOUT> struct OutRegs<FX_PREC_F16, G, NOUT> {
 // (low, high) = (earlier, later).  FFT gi of the group enters half gi & 1 of
 // the last word, and before each even FFT the words move down by one: one
 // word move per output and FFT pair instead of a half-word shift per FFT.
-template <int G, int NOUT> struct OutRegs<FX_PREC_F8, G, NOUT> {
-    uint32_t v[G / 2][NOUT];
+// With HALF the chain is W = G/4 words (a half row), emptied half-way.
+template <int G, int NOUT, bool HALF> struct OutRegs<FX_PREC_F8, G, NOUT, HALF> {
+    static constexpr int W = HALF ? G / 4 : G / 2;
+    uint32_t v[W][NOUT];
     __device__ __forceinline__ void shift(int gi) {
         if (gi & 1) return;
 #pragma unroll
         for (int u = 0; u < NOUT; u++)
 #pragma unroll
-            for (int w = 0; w + 1 < G / 2; w++) v[w][u] = v[w + 1][u];
+            for (int w = 0; w + 1 < W; w++) v[w][u] = v[w + 1][u];
     }
     __device__ __forceinline__ void zero_last(int gi) {
 #pragma unroll
-        for (int u = 0; u < NOUT; u++) v[G / 2 - 1][u] = (gi & 1) ? (v[G / 2 - 1][u] & 0xffffu) : 0u;
+        for (int u = 0; u < NOUT; u++) v[W - 1][u] = (gi & 1) ? (v[W - 1][u] & 0xffffu) : 0u;
     }
     __device__ __forceinline__ void put_last(int u, float2 x, float si, int gi) {
         const float a = fminf(fmaxf(x.x * si, -448.0f), 448.0f);
         const float b = fminf(fmaxf(x.y * si, -448.0f), 448.0f);
         if (gi & 1)
-            v[G / 2 - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, (int)v[G / 2 - 1][u], true);
+            v[W - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, (int)v[W - 1][u], true);
         else
-            v[G / 2 - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+            v[W - 1][u] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
     }
 };
 
@@ -981,7 +987,24 @@ void fx_fstage_kernel(FStageArgs args)
     }
 
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
-    OutRegs<PREC, G, NOUT> outr;
+    // FP8 with complex input at F >= 8192 (16 channels per thread): half rows,
+    // so the output chain takes half the registers (the kernels spilled 72-148
+    // bytes per lane holding whole rows)
+    constexpr bool F8HALF = PREC == FX_PREC_F8 && CPLX && NOUT >= 16;
+    OutRegs<PREC, G, NOUT, F8HALF> outr;
+    // half h of every output row: 8 bytes, two words of the chain
+    [[maybe_unused]] auto store_half = [&](int h, int t) {
+        if constexpr (F8HALF) {
+#pragma unroll
+            for (int m = 0; m < PERL; m++)
+#pragma unroll
+                for (int r = 0; r < RO; r++) {
+                    const int ou = m * RO + r;
+                    const long long row = xrow0 + (long long)out_chan<Gm>(t, m, r) * cpitch;
+                    reinterpret_cast<uint2*>(args.out)[row * 2 + h] = make_uint2(outr.v[0][ou], outr.v[1][ou]);
+                }
+        }
+    };
 
     // The next FFT's rotator tables and packed-byte window are loaded into
     // registers while the current FFT runs; the delay entry (offset, valid)
@@ -1091,6 +1114,9 @@ void fx_fstage_kernel(FStageArgs args)
         // out of the loop and held in registers across all passes.
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid >= 0 && tid < NT);
+        if constexpr (F8HALF) {
+            if (gi == G / 2) store_half(0, tid);  // FFTs 0 .. G/2-1 are done
+        }
         outr.shift(gi);
         const int fft = grp * G + gi;             // FFT of this launch
         __syncthreads();                          // previous FFT done with LDS
@@ -1405,6 +1431,9 @@ void fx_fstage_kernel(FStageArgs args)
                 static_assert(G == 2, "FP32 rows are (re0, re1, im0, im1)");
                 float4* dst = reinterpret_cast<float4*>(args.out) + row;
                 *dst = make_float4(outr.v[0][ou].x, outr.v[1][ou].x, outr.v[0][ou].y, outr.v[1][ou].y);
+            } else if constexpr (F8HALF) {
+                static_assert(G == 8, "two half rows of four FFTs");
+                reinterpret_cast<uint2*>(args.out)[row * 2 + 1] = make_uint2(outr.v[0][ou], outr.v[1][ou]);
             } else {
                 uint4* dst = reinterpret_cast<uint4*>(args.out) + row * (G / 8);
 #pragma unroll
