@@ -214,11 +214,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         }
         okm = (SI < nblk && SJ < nblk) ? 1u : 0u;
     } else if constexpr (SUBT8) {
+        // off the diagonal: row wave/2, columns 2*(wave%2) + j.  On a diagonal
+        // pair the ten blocks I <= J as four 1x2 and two 1x1 sub-tiles, placed
+        // so that the SIMDs (wave % 4) carry 3, 3, 2 and 2 blocks
+        // (waves 0-7: rows 0 0 1 2 1 3 - -, first columns 0 2 2 2 1 3 - -,
+        // widths 2 2 2 2 1 1 0 0)
+        const int sh = 4 * wave;
+        const int ri = diag ? (0x00312100 >> sh) & 15 : wave >> 1;
+        const int c0 = diag ? (0x00312220 >> sh) & 15 : 2 * (wave & 1);
+        const int nw = diag ? (0x00112222 >> sh) & 15 : 2;
 #pragma unroll
         for (int t = 0; t < NBW; t++) {
-            const int bi = wave >> 1, bj = 2 * (wave & 1) + (t & 1);
+            const int bi = ri, bj = c0 + (t & 1);
             const int I = SI * SB + bi, J = SJ * SB + bj;
-            const bool ok = t < 2 && (!diag || bi <= bj) && I < nblk && J < nblk;
+            const bool ok = t < nw && I < nblk && J < nblk;
             bI[t] = bi;
             bJ[t] = bj;
             okm |= (unsigned)ok << t;
@@ -336,7 +345,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         for (int sub = 0; sub < NSUB; sub++) {
         if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
         if constexpr (PREC == FX_PREC_F16 && SUBT8) {
-            const int r0 = (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
+            const int r0 = bI[0] * FX_BLK + row, c0 = bJ[0] * FX_BLK + row;
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
                 if (!(okm & 3u)) break;
@@ -406,7 +415,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
                 }
             }
         } else if constexpr (SUBT8 && PREC == FX_PREC_F8) {
-            const int r0 = (wave >> 1) * FX_BLK + row, c0 = 2 * (wave & 1) * FX_BLK + row;
+            const int r0 = bI[0] * FX_BLK + row, c0 = bJ[0] * FX_BLK + row;
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ks++) {
                 if (!(okm & 3u)) break;
