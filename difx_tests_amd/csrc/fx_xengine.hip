@@ -162,9 +162,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     constexpr int NELEM = NPL * PWL * KC;                // 16-byte rows per chunk
     constexpr int NLD = (NSUB * NELEM + 64 * NW - 1) / (64 * NW);
     constexpr bool PART = (NSUB * NELEM) % (64 * NW) != 0;   // threads past the step's rows load nothing
-    // [stage][chunk][panel][group][signal]; H16 keeps two panels' space, which
-    // its partial-sum reduction reuses
-    __shared__ __align__(16) uint4 lds[2][NSUB][NTL ? 1 : 2][KC][PW];
+    // [stage][chunk][panel][group][signal], the panels and rows staged; H16's
+    // partial-sum reduction reuses it
+    __shared__ __align__(16) uint4 lds[2][NSUB][NPL][KC][PWL];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
     // the launch has 64*NW threads: the panel of a load slot is then a
@@ -339,6 +339,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     __syncthreads();
     // KSPLIT: K step k of the launch belongs to wave k % NW
     auto mine = [&](int k) { return !KSPLIT || (k & (NW - 1)) == wave; };
+    // the wave index as a wave-uniform value, for LDS addresses formed from it
+    const int wvu = __builtin_amdgcn_readfirstlane(wave);
     // the MFMAs of the chunks of step sp from LDS stage stg
     auto compute = [&](int sp, int stg) {
 #pragma unroll
@@ -377,21 +379,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
                 }
             }
         } else if constexpr (PREC == FX_PREC_F32 && H16) {
-            // one K step per row (2 FFTs): lane l holds element l / 16 of row
-            // l % 16 in the order (re0, re1, im0, im1), and of A' = (im, -re)
-            // in the order (im0, im1, -re0, -re1)
-#pragma unroll
-            for (int g = 0; g < KC; g++) {
-                if (!mine((sp * NSUB + sub) * KC + g)) continue;
-                if (okm & 1u) {
-                    const float4 A = as_float4(lds[stg][sub][0][g][r16]);
-                    const float4 B = as_float4(lds[stg][sub][pb][g][r16]);
-                    const float av = k4 == 0 ? A.x : k4 == 1 ? A.y : k4 == 2 ? A.z : A.w;
-                    const float cv = k4 == 0 ? A.z : k4 == 1 ? A.w : k4 == 2 ? -A.x : -A.y;
-                    const float bv = k4 == 0 ? B.x : k4 == 1 ? B.y : k4 == 2 ? B.z : B.w;
-                    acr16 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acr16, 0, 0, 0);
-                    aci16 = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, bv, aci16, 0, 0, 0);
-                }
+            // one K step per row (2 FFTs), and with KC == NW the K step of
+            // group g of every chunk is wave g's: lane l reads element l / 16
+            // of row l % 16 of its wave's group, in the order (re0, re1, im0,
+            // im1), and the element of A' = (im, -re), in the order (im0, im1,
+            // -re0, -re1).  The one block is diagonal: B = A.
+            static_assert(KC == NW, "one group of each chunk per wave");
+            if (okm & 1u) {
+                const float* rp = reinterpret_cast<const float*>(&lds[stg][sub][0][wvu][r16]);
+                const float av = rp[k4];
+                const float cw = rp[(k4 + 2) & 3];
+                const float cv = __int_as_float(__float_as_int(cw) ^ (k4 >= 2 ? (int)0x80000000 : 0));
+                acr16 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, av, acr16, 0, 0, 0);
+                aci16 = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, av, aci16, 0, 0, 0);
             }
         } else if constexpr (PREC == FX_PREC_F32) {
             // rows (re0, re1, im0, im1); K step = 1 FFT, lanes 0-31 re, 32-63 im
@@ -496,9 +496,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     if constexpr (KSPLIT) {
         // wave 0 adds the other waves' partial sums, one wave at a time
         // through the staging LDS (free after the loop's last barrier; it
-        // holds at least one wave's 2 x 32 x 32 floats); the order is fixed,
+        // holds at least one wave's 2 x 32 x 32 floats, or 2 x 16 x 16 for a
+        // 16x16 quarter); the order is fixed,
         // so the result is the same on every run
-        static_assert(sizeof(lds) >= 2 * 16 * 64 * sizeof(float), "staging LDS holds one wave's partial sums");
+        static_assert(sizeof(lds) >= 2 * (H16 ? 4 : 16) * 64 * sizeof(float), "staging LDS holds one wave's partial sums");
         float* red = reinterpret_cast<float*>(&lds[0][0][0][0][0]);
         for (int w = 1; w < NW; w++) {
             if (wave == w) {
@@ -702,7 +703,7 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
     const int npan = (a.p.nblk + SB - 1) / SB;
     const int npairs = npan * (npan + 1) / 2;
     const dim3 grid(a.p.nchan * npairs), block(64 * WAVES);
-    constexpr int DEEP = PREC == FX_PREC_F8 ? 2 : 4;    // 32 KB of LDS either way
+    constexpr int DEEP = PREC == FX_PREC_F8 ? 2 : 4;    // 32 KB of LDS either way (8 KB for 16x16 blocks)
     switch (SB) {
     case 4:
         // FP16 with one panel pair per channel (at most 64 antennas): every
