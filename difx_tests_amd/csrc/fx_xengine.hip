@@ -347,16 +347,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         for (int sub = 0; sub < NSUB; sub++) {
         if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
         if constexpr (PREC == FX_PREC_F16 && SUBT8) {
-            const int r0 = bI[0] * FX_BLK + row, c0 = bJ[0] * FX_BLK + row;
+            // every fragment of the chunk is read before its MFMAs, so the
+            // LDS reads overlap (a slot without a block reads the first
+            // column again, inside the panel)
+            if (!(okm & 3u)) continue;
+            const int r0 = bI[0] * FX_BLK + row;
+            int cj[2];
+#pragma unroll
+            for (int j = 0; j < 2; j++) cj[j] = ((okm >> j) & 1u ? bJ[j] : bJ[0]) * FX_BLK + row;
+            uint4 A[KC / 2], B[KC / 2][2];
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
-                if (!(okm & 3u)) break;
-                const uint4 A = lds[stg][sub][0][2 * ks + kg][r0];
-                const half8 ha = as_half8(A), hc = as_half8(conj_swap_f16(A));
+                A[ks] = lds[stg][sub][0][2 * ks + kg][r0];
+#pragma unroll
+                for (int j = 0; j < 2; j++) B[ks][j] = lds[stg][sub][pb][2 * ks + kg][cj[j]];
+            }
+#pragma unroll
+            for (int ks = 0; ks < KC / 2; ks++) {
+                const half8 ha = as_half8(A[ks]), hc = as_half8(conj_swap_f16(A[ks]));
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
                     if ((okm >> j) & 1u) {
-                        const half8 hb = as_half8(lds[stg][sub][pb][2 * ks + kg][c0 + j * FX_BLK]);
+                        const half8 hb = as_half8(B[ks][j]);
                         accr[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ha, hb, accr[j], 0, 0, 0);
                         acci[j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hc, hb, acci[j], 0, 0, 0);
                     }
