@@ -164,7 +164,12 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     constexpr bool PART = (NSUB * NELEM) % (64 * NW) != 0;   // threads past the step's rows load nothing
     // [stage][chunk][panel][group][signal], the panels and rows staged; H16's
     // partial-sum reduction reuses it
-    __shared__ __align__(16) uint4 lds[2][NSUB][NPL][KC][PWL];
+    // (direct-to-LDS kernels keep their second stage in an array of its own,
+    // so that the compiler sees that loads into one stage do not alias reads
+    // of the other)
+    constexpr bool DMA = NW == 8 && SB == 4 && !NTL;
+    __shared__ __align__(16) uint4 lds[DMA ? 1 : 2][NSUB][NPL][KC][PWL];
+    __shared__ __align__(16) uint4 lds_b[DMA ? NSUB : 1][DMA ? NPL : 1][DMA ? KC : 1][DMA ? PWL : 1];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
     // the launch has 64*NW threads: the panel of a load slot is then a
@@ -318,6 +323,33 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         }
     };
 
+    // Eight waves with one chunk in flight: each chunk goes from global memory
+    // straight into its LDS stage (global_load_lds_dwordx4: lane i of a wave
+    // writes the 16 bytes at the wave's LDS base + 16 i, and the elements e of
+    // a wave are 64 consecutive rows of the stage), so no staging registers and
+    // no LDS stores; a step waits for them before its closing barrier.
+    static_assert(!DMA || (SUBT8 && PF == 1), "direct-to-LDS staging: eight waves, one chunk in flight");
+    typedef __attribute__((address_space(3))) void lds_void;
+    auto dma = [&](int sp, auto& stage) {
+#pragma unroll
+        for (int l = 0; l < NLD; l++) {
+            const int e = tid + l * 64 * NW;
+            if (PART && e >= NSUB * NELEM) continue;
+            const int sub = e / NELEM, e2 = e - sub * NELEM;
+            const int pan = NPL == 1 ? 0 : e2 / (KC * PWL), rem = e2 - pan * (KC * PWL);
+            const int g = rem / PWL, sgl = rem - g * PWL;
+            const int sig = (pan ? SJ : SI) * PW + sgl;
+            const int chunk = sp * NSUB + sub;
+            const int chc = chunk < nchunk ? chunk : nchunk - 1;
+            const char* cb = reinterpret_cast<const char*>(gbase + chc * cstride);
+            const unsigned boff = (unsigned)(g * Spad + (sig < Spad ? sig : Spad - 1)) * 16u;
+            if (NPL == 1 || NSUB > 1 || !(diag && pan)) {
+                uint4* wb = &stage[0][0][0][0] + (e - lane);
+                __builtin_amdgcn_global_load_lds(cb + boff, (lds_void*)wb, 16, 0, NTL ? 2 : 0);
+            }
+        }
+    };
+
     static_assert(!H16 || (PREC == FX_PREC_F32 && SB == 1), "16x16 blocks: FP32, one block per pair");
     f32x16 accr[NBW], acci[NBW];
 #pragma unroll
@@ -331,8 +363,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     const int pb = (NPL == 1 || diag) ? 0 : 1;            // panel of the B operand
 
     if (nstep > 0) {
-        gload(0, st[0]);
-        sstore(0, st[0]);
+        if constexpr (DMA) {
+            dma(0, lds[0]);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        } else {
+            gload(0, st[0]);
+            sstore(0, st[0]);
+        }
     }
     // (launches have at least one chunk; past the last one, the last again)
     if constexpr (PF == 2) gload(1, st[1]);
@@ -342,7 +379,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     // the wave index as a wave-uniform value, for LDS addresses formed from it
     const int wvu = __builtin_amdgcn_readfirstlane(wave);
     // the MFMAs of the chunks of step sp from LDS stage stg
-    auto compute = [&](int sp, int stg) {
+    auto compute = [&](int sp, auto& L) {
 #pragma unroll
         for (int sub = 0; sub < NSUB; sub++) {
         if (sp * NSUB + sub >= nchunk) break;             // last step of a chunk count not a multiple of NSUB
@@ -358,9 +395,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             uint4 A[KC / 2], B[KC / 2][2];
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
-                A[ks] = lds[stg][sub][0][2 * ks + kg][r0];
+                A[ks] = L[sub][0][2 * ks + kg][r0];
 #pragma unroll
-                for (int j = 0; j < 2; j++) B[ks][j] = lds[stg][sub][pb][2 * ks + kg][cj[j]];
+                for (int j = 0; j < 2; j++) B[ks][j] = L[sub][pb][2 * ks + kg][cj[j]];
             }
 #pragma unroll
             for (int ks = 0; ks < KC / 2; ks++) {
@@ -381,8 +418,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
-                        const uint4 A = lds[stg][sub][0][2 * ks + kg][bI[t] * FX_BLK + row];
-                        const uint4 B = lds[stg][sub][pb][2 * ks + kg][bJ[t] * FX_BLK + row];
+                        const uint4 A = L[sub][0][2 * ks + kg][bI[t] * FX_BLK + row];
+                        const uint4 B = L[sub][pb][2 * ks + kg][bJ[t] * FX_BLK + row];
                         const half8 hb = as_half8(B);
                         accr[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(A), hb, accr[t], 0, 0, 0);
                         acci[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(conj_swap_f16(A)), hb, acci[t], 0,
@@ -398,7 +435,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             // -re0, -re1).  The one block is diagonal: B = A.
             static_assert(KC == NW, "one group of each chunk per wave");
             if (okm & 1u) {
-                const float* rp = reinterpret_cast<const float*>(&lds[stg][sub][0][wvu][r16]);
+                const float* rp = reinterpret_cast<const float*>(&L[sub][0][wvu][r16]);
                 const float av = rp[k4];
                 const float cw = rp[(k4 + 2) & 3];
                 const float cv = __int_as_float(__float_as_int(cw) ^ (k4 >= 2 ? (int)0x80000000 : 0));
@@ -413,8 +450,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
-                        const float4 A = as_float4(lds[stg][sub][0][g][bI[t] * FX_BLK + row]);
-                        const float4 B = as_float4(lds[stg][sub][pb][g][bJ[t] * FX_BLK + row]);
+                        const float4 A = as_float4(L[sub][0][g][bI[t] * FX_BLK + row]);
+                        const float4 B = as_float4(L[sub][pb][g][bJ[t] * FX_BLK + row]);
                         const float av[2] = {kg ? A.z : A.x, kg ? A.w : A.y};
                         const float cv[2] = {kg ? -A.x : A.z, kg ? -A.y : A.w};
                         const float bv[2] = {kg ? B.z : B.x, kg ? B.w : B.y};
@@ -431,8 +468,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ks++) {
                 if (!(okm & 3u)) break;
-                const uint4 a0 = lds[stg][sub][0][4 * ks + 2 * kg][r0];
-                const uint4 a1 = lds[stg][sub][0][4 * ks + 2 * kg + 1][r0];
+                const uint4 a0 = L[sub][0][4 * ks + 2 * kg][r0];
+                const uint4 a1 = L[sub][0][4 * ks + 2 * kg + 1][r0];
                 const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
                                   (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
                 i32x8 cv;
@@ -441,8 +478,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
                     if ((okm >> j) & 1u) {
-                        const uint4 b0 = lds[stg][sub][pb][4 * ks + 2 * kg][c0 + j * FX_BLK];
-                        const uint4 b1 = lds[stg][sub][pb][4 * ks + 2 * kg + 1][c0 + j * FX_BLK];
+                        const uint4 b0 = L[sub][pb][4 * ks + 2 * kg][c0 + j * FX_BLK];
+                        const uint4 b1 = L[sub][pb][4 * ks + 2 * kg + 1][c0 + j * FX_BLK];
                         const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w,
                                           (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
                         accr[j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, accr[j], 0, 0, 0, 127, 0, 127);
@@ -458,10 +495,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 #pragma unroll
                 for (int t = 0; t < NBW; t++) {
                     if ((okm >> t) & 1u) {
-                        const uint4 a0 = lds[stg][sub][0][4 * ks + 2 * kg][bI[t] * FX_BLK + row];
-                        const uint4 a1 = lds[stg][sub][0][4 * ks + 2 * kg + 1][bI[t] * FX_BLK + row];
-                        const uint4 b0 = lds[stg][sub][pb][4 * ks + 2 * kg][bJ[t] * FX_BLK + row];
-                        const uint4 b1 = lds[stg][sub][pb][4 * ks + 2 * kg + 1][bJ[t] * FX_BLK + row];
+                        const uint4 a0 = L[sub][0][4 * ks + 2 * kg][bI[t] * FX_BLK + row];
+                        const uint4 a1 = L[sub][0][4 * ks + 2 * kg + 1][bI[t] * FX_BLK + row];
+                        const uint4 b0 = L[sub][pb][4 * ks + 2 * kg][bJ[t] * FX_BLK + row];
+                        const uint4 b1 = L[sub][pb][4 * ks + 2 * kg + 1][bJ[t] * FX_BLK + row];
                         const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
                                           (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
                         i32x8 cv;
@@ -477,11 +514,24 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         }
         }   // sub
     };
-    if constexpr (PF == 1) {
+    if constexpr (DMA) {
+        // stage 0 is lds[0], stage 1 lds_b: two steps per trip
+        for (int sp = 0; sp < nstep; sp += 2) {
+            if (sp + 1 < nstep) dma(sp + 1, lds_b);
+            compute(sp, lds[0]);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (sp + 1 >= nstep) break;
+            if (sp + 2 < nstep) dma(sp + 2, lds[0]);
+            compute(sp + 1, lds_b);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+        }
+    } else if constexpr (PF == 1) {
         for (int sp = 0; sp < nstep; sp++) {
             const int stg = sp & 1;
             if (sp + 1 < nstep) gload(sp + 1, st[0]);
-            compute(sp, stg);
+            compute(sp, lds[stg]);
             if (sp + 1 < nstep) sstore(stg ^ 1, st[0]);
             __syncthreads();
         }
@@ -495,7 +545,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         auto step = [&](int sp, auto par) {
             constexpr int P = decltype(par)::value;
             gload(sp + 2, st[P]);
-            compute(sp, P);
+            compute(sp, lds[P]);
             sstore(P ^ 1, st[P ^ 1]);
             __syncthreads();
         };

@@ -106,12 +106,15 @@ def test_graph_replays_against_eager():
         e.close()
 
 
-# 70 antennas: 5 signal blocks, partial last panels; 48 and 64 antennas: 3 and
-# 4 blocks, one panel pair per channel at tile 4 (the FP16 kernel that stages
-# one panel and keeps two K chunks in flight), over K loops of 1, 3, 4 and 5
-# chunks of 16 FFTs
-@pytest.mark.parametrize("prec,nant,nf", [("f32", 70, 4), ("f16", 70, 4), ("f32v", 70, 4), ("f16", 48, 4),
-                                          ("f16", 48, 48), ("f16", 64, 64), ("f16", 64, 80)])
+# 70 antennas: 5 signal blocks, partial last panels (at tile 4 the FP16 and
+# FP8 kernels that load K chunks straight into two alternating LDS stages,
+# over K loops of 1 to 5 chunks: 16 FFTs per FP16 chunk, 64 per FP8 chunk);
+# 48 and 64 antennas: 3 and 4 blocks, one panel pair per channel at tile 4
+# (the FP16 kernel that stages one panel and keeps two K chunks in flight),
+# over K loops of 1, 3, 4 and 5 chunks of 16 FFTs
+@pytest.mark.parametrize("prec,nant,nf", [("f32", 70, 4), ("f16", 70, 4), ("f32v", 70, 4), ("f16", 70, 48),
+                                          ("f16", 70, 80), ("f8", 70, 64), ("f8", 70, 128), ("f8", 70, 192),
+                                          ("f16", 48, 4), ("f16", 48, 48), ("f16", 64, 64), ("f16", 64, 80)])
 def test_xtile_does_not_change_results(prec, nant, nf):
     nchan = 256
     p, data, delays, fileoff = _subband(nchan, nant, 16e6, nf, 77)
