@@ -111,12 +111,14 @@ __device__ __forceinline__ void store_block(const XArgs& a, int c, int bp, int I
     }
 }
 
-// The top-left 16x16 quarter of block (0, 0) of channel c from a 16x16 MFMA
-// accumulator: rows 4 * (lane / 16) + v, column lane % 16.
-__device__ __forceinline__ void store_block16(const XArgs& a, int c, int lane, const f32x4& re, const f32x4& im)
+// The 16x16 quarter (qi, qj) of block (0, 0) of channel c from a 16x16 MFMA
+// accumulator: rows 4 * (lane / 16) + v, column lane % 16 of the quarter.
+__device__ __forceinline__ void store_block16(const XArgs& a, int c, int lane, const f32x4& re, const f32x4& im,
+                                              int qi = 0, int qj = 0)
 {
     const int n = lane & 15, m0 = 4 * (lane >> 4);
-    float2* dst = a.vacc + ((long long)c * a.p.nbp + fx_bp_index(0, 0, a.p.nblk)) * (FX_BLK * FX_BLK);
+    float2* dst = a.vacc + ((long long)c * a.p.nbp + fx_bp_index(0, 0, a.p.nblk)) * (FX_BLK * FX_BLK) +
+                  16 * qi * FX_BLK + 16 * qj;
     float2 old[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) old[v] = a.accumulate ? dst[(m0 + v) * FX_BLK + n] : make_float2(0.0f, 0.0f);
@@ -141,13 +143,16 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // pair covers every signal, so that no other workgroup reads the same rows.
 // H16 (FP32, SB = 1, at most 16 signals): the one block is formed as its
 // top-left 16x16 quarter on v_mfma_f32_16x16x4f32, a quarter of the MFMA
-// work of a 32x32 block for the same K.
+// work of a 32x32 block for the same K.  Q3 (FP32, one block of 17 to 32
+// signals): its quarters (0, 0), (0, 1) and (1, 1) on v_mfma_f32_16x16x4f32,
+// three quarters of the work; quarter (1, 0) holds the conjugates of (0, 1),
+// and no visibility is read from it (the export reads i <= j).
 // NTL also means one panel pair per channel (every signal in one diagonal
 // super-tile): one panel is staged, each thread issues one branch-free load
 // per chunk, and the loads run two chunks ahead in two register sets (the
 // waits before the LDS stores then cover only the older chunk; a conditional
 // load would make the compiler wait for every load in flight).
-template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false, bool H16 = false>
+template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false, bool H16 = false, bool Q3 = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
@@ -155,8 +160,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     // rows staged per group and panel, and panels staged: H16 stages only the
     // 16 rows its quarter block reads, of its one (diagonal) panel; NTL the
     // one (diagonal) panel of its grid
+    constexpr bool QB = H16 || Q3;                       // 16x16 quarters
+    constexpr int NQ = Q3 ? 3 : 1;
     constexpr int PWL = H16 ? FX_BLK / 2 : PW;
-    constexpr int NPL = (H16 || NTL) ? 1 : 2;
+    constexpr int NPL = (QB || NTL) ? 1 : 2;
     constexpr int PF = NTL ? 2 : 1;                      // chunks in flight per thread
     static_assert(PF == 1 || NSUB == 1, "two chunks in flight: one chunk per step");
     constexpr int NELEM = NPL * PWL * KC;                // 16-byte rows per chunk
@@ -283,7 +290,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     // blocks beyond nblk, which okm excludes), and with one chunk per step a
     // diagonal workgroup skips its second panel (a workgroup- or wave-uniform
     // branch; that LDS is not read).
-    static_assert(H16 || NELEM % (64 * NW) == 0, "whole loads per thread");
+    static_assert(QB || NELEM % (64 * NW) == 0, "whole loads per thread");
     uint4 st[PF][NLD];
 #pragma unroll
     for (int q = 0; q < PF; q++)
@@ -350,14 +357,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         }
     };
 
-    static_assert(!H16 || (PREC == FX_PREC_F32 && SB == 1), "16x16 blocks: FP32, one block per pair");
+    static_assert(!QB || (PREC == FX_PREC_F32 && SB == 1 && !(H16 && Q3)), "16x16 blocks: FP32, one block per pair");
     f32x16 accr[NBW], acci[NBW];
 #pragma unroll
     for (int t = 0; t < NBW; t++) {
         accr[t] = (f32x16){};
         acci[t] = (f32x16){};
     }
-    f32x4 acr16 = {}, aci16 = {};
+    f32x4 acr16[NQ], aci16[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        acr16[q] = (f32x4){};
+        aci16[q] = (f32x4){};
+    }
     const int row = lane & 31, kg = lane >> 5;
     const int r16 = lane & 15, k4 = lane >> 4;
     const int pb = (NPL == 1 || diag) ? 0 : 1;            // panel of the B operand
@@ -439,8 +451,26 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
                 const float av = rp[k4];
                 const float cw = rp[(k4 + 2) & 3];
                 const float cv = __int_as_float(__float_as_int(cw) ^ (k4 >= 2 ? (int)0x80000000 : 0));
-                acr16 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, av, acr16, 0, 0, 0);
-                aci16 = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, av, aci16, 0, 0, 0);
+                acr16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, av, acr16[0], 0, 0, 0);
+                aci16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, av, aci16[0], 0, 0, 0);
+            }
+        } else if constexpr (PREC == FX_PREC_F32 && Q3) {
+            // as H16, for rows r16 (quarter row 0) and 16 + r16 (quarter row
+            // 1): quarter (qi, qj) is rows qi against rows qj
+            static_assert(KC == NW, "one group of each chunk per wave");
+            if (okm & 1u) {
+                const float* rp0 = reinterpret_cast<const float*>(&L[sub][0][wvu][r16]);
+                const float* rp1 = reinterpret_cast<const float*>(&L[sub][0][wvu][16 + r16]);
+                const int sg = k4 >= 2 ? (int)0x80000000 : 0;
+                const float a0 = rp0[k4], a1 = rp1[k4];
+                const float c0 = __int_as_float(__float_as_int(rp0[(k4 + 2) & 3]) ^ sg);
+                const float c1 = __int_as_float(__float_as_int(rp1[(k4 + 2) & 3]) ^ sg);
+                acr16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, acr16[0], 0, 0, 0);
+                aci16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(c0, a0, aci16[0], 0, 0, 0);
+                acr16[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a1, acr16[1], 0, 0, 0);
+                aci16[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(c0, a1, aci16[1], 0, 0, 0);
+                acr16[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a1, acr16[2], 0, 0, 0);
+                aci16[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(c1, a1, aci16[2], 0, 0, 0);
             }
         } else if constexpr (PREC == FX_PREC_F32) {
             // rows (re0, re1, im0, im1); K step = 1 FFT, lanes 0-31 re, 32-63 im
@@ -561,15 +591,17 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         // holds at least one wave's 2 x 32 x 32 floats, or 2 x 16 x 16 for a
         // 16x16 quarter); the order is fixed,
         // so the result is the same on every run
-        static_assert(sizeof(lds) >= 2 * (H16 ? 4 : 16) * 64 * sizeof(float), "staging LDS holds one wave's partial sums");
+        static_assert(sizeof(lds) >= 2 * (QB ? 4 : 16) * 64 * sizeof(float), "staging LDS holds one wave's partial sums");
         float* red = reinterpret_cast<float*>(&lds[0][0][0][0][0]);
-        for (int w = 1; w < NW; w++) {
+#pragma unroll
+        for (int u = 0; u < (QB ? NQ : 1) * (NW - 1); u++) {
+            const int w = 1 + u % (NW - 1), q = QB ? u / (NW - 1) : 0;   // quarter q of wave w
             if (wave == w) {
-                if constexpr (H16) {
+                if constexpr (QB) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        red[v * 64 + lane] = acr16[v];
-                        red[(4 + v) * 64 + lane] = aci16[v];
+                        red[v * 64 + lane] = acr16[q][v];
+                        red[(4 + v) * 64 + lane] = aci16[q][v];
                     }
                 } else {
 #pragma unroll
@@ -581,11 +613,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             }
             __syncthreads();
             if (wave == 0) {
-                if constexpr (H16) {
+                if constexpr (QB) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        acr16[v] += red[v * 64 + lane];
-                        aci16[v] += red[(4 + v) * 64 + lane];
+                        acr16[q][v] += red[v * 64 + lane];
+                        aci16[q][v] += red[(4 + v) * 64 + lane];
                     }
                 } else {
 #pragma unroll
@@ -600,7 +632,15 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         if (wave != 0) okm = 0;
     }
     if constexpr (H16) {
-        if (okm & 1u) store_block16(a, c, lane, acr16, aci16);
+        if (okm & 1u) store_block16(a, c, lane, acr16[0], aci16[0]);
+        return;
+    }
+    if constexpr (Q3) {
+        if (okm & 1u) {
+            store_block16(a, c, lane, acr16[0], aci16[0], 0, 0);
+            store_block16(a, c, lane, acr16[1], aci16[1], 0, 1);
+            store_block16(a, c, lane, acr16[2], aci16[2], 1, 1);
+        }
         return;
     }
     const float* sc = (PREC == FX_PREC_F8) ? a.scale + (long long)c * Spad : nullptr;
@@ -789,6 +829,13 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
                     fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, true><<<grid, block, 0, s>>>(a);
                 else
                     fx_xengine_kernel<PREC, 1, 1, WAVES, false, true><<<grid, block, 0, s>>>(a);
+                break;
+            }
+            if (a.p.nblk == 1) {                          // 17 to 32 signals: three 16x16 quarters
+                if ((long long)a.p.nchan * npairs <= kSmallGrid)
+                    fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, false, true><<<grid, block, 0, s>>>(a);
+                else
+                    fx_xengine_kernel<PREC, 1, 1, WAVES, false, false, true><<<grid, block, 0, s>>>(a);
                 break;
             }
         }
