@@ -1458,10 +1458,51 @@ static int autotune_run(fx_context* c, int numffts, int reps, double* report, Tu
     // steady state, neither of which timing the X-engine alone shows.  Tiles
     // within 25% of the best alone, k = 1, 2, 4, ... (up to the FFT capacity),
     // round-robin over the rounds with each pair's best kept; then the
-    // smallest k whose best tile is within 1% of the best time per subint (a
-    // larger k only adds buffer memory and latency) and that tile.
-    auto step_ms = [&](int k, double* ms) {
+    // smallest k whose best tile is within 0.2% of the best time per subint (a
+    // larger k only adds buffer memory and latency; a 1% margin kept 4 subints
+    // per launch at 16 antennas x 2048 channels where 16 ran 7% faster) and
+    // that tile.
+    int counts[8], nk = 0;
+    for (int k = 1; nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) counts[nk++] = k;
+    // k subints as callers with a buffer of several subints run them: one
+    // delay / F-stage launch over their FFTs (here the trial's FFT list k
+    // times, one upload of the longest list serving every k) and one X-engine
+    // launch over the slots; where the input capacity does not hold k
+    // subints, one F-stage launch per subint
+    const FxPlan& pp = c->p;
+    auto batch_fits = [&](int k) {
+        const long long n = (long long)k * numffts;
+        const long long need = pp.iscomplex ? 4LL * n * pp.F : (n * pp.F + 1) / 2;
+        return k > 1 && n <= c->max_ffts && need <= c->cap_bytes && (n * pp.F) <= 0x7fffffffLL - 2LL * pp.F;
+    };
+    FStageCall fk[8];
+    int kmax = 0;
+    for (int i = 0; i < nk; i++)
+        if (batch_fits(counts[i])) kmax = counts[i];
+    if (rc == FX_OK && kmax > 1) {
+        std::vector<int> itk((size_t)kmax * numffts);
+        for (size_t q = 0; q < itk.size(); q++) itk[q] = (int)(q % numffts);
+        for (int i = 0; rc == FX_OK && i < nk; i++) {
+            if (!batch_fits(counts[i])) continue;
+            rc = fstage_check(c, counts[i] * numffts, itk.data(), 1, 0, &fk[i]);
+        }
+        FStageCall fmax;
+        if (rc == FX_OK) rc = fstage_check(c, kmax * numffts, itk.data(), 1, 0, &fmax);
+        if (rc == FX_OK) rc = iter_upload(c, itk.data(), fmax, c->stream);
+        if (rc == FX_OK) rc = hipStreamSynchronize(c->stream) == hipSuccess ? FX_OK : fail(FX_EHIP, "stream sync");
+        for (int i = 0; i < nk; i++) {
+            fk[i].contig = 0;                             // the uploaded list's prefix
+            fk[i].it0 = 0;
+        }
+    }
+    auto step_ms = [&](int i, double* ms) {
+        const int k = counts[i];
         return tune_time(c, reps, ms, [&] {
+            if (batch_fits(k)) {
+                const int r = fstage_enqueue(c, fk[i], c->stream);
+                if (r) return r;
+                return xstage_enqueue(c, 0, fk[i].npad, c->stream);
+            }
             for (int s = 0; s < k; s++) {
                 FStageCall fs = f;
                 fs.fft0 = s * f.npad;
@@ -1471,8 +1512,6 @@ static int autotune_run(fx_context* c, int numffts, int reps, double* report, Tu
             return xstage_enqueue(c, 0, k * f.npad, c->stream);
         });
     };
-    int counts[8], nk = 0;
-    for (int k = 1; nk < 8 && (long long)k * f.npad <= c->max_ffts; k *= 2) counts[nk++] = k;
     double per_sub[3][8];
     for (int t = 0; t < 3; t++)
         for (int i = 0; i < 8; i++) per_sub[t][i] = 1e30;
@@ -1482,7 +1521,7 @@ static int autotune_run(fx_context* c, int numffts, int reps, double* report, Tu
             c->xsb = tiles[t];
             for (int i = 0; rc == FX_OK && i < nk; i++) {
                 double ms = 0.0;
-                rc = step_ms(counts[i], &ms);
+                rc = step_ms(i, &ms);
                 if (rc == FX_OK) per_sub[t][i] = std::min(per_sub[t][i], ms / counts[i]);
             }
         }
@@ -1495,7 +1534,7 @@ static int autotune_run(fx_context* c, int numffts, int reps, double* report, Tu
         for (int i = 0; i < nk && !found; i++) {
             int tb = -1;
             for (int t = 0; t < 3; t++)
-                if (per_sub[t][i] <= 1.01 * m && (tb < 0 || per_sub[t][i] < per_sub[tb][i])) tb = t;
+                if (per_sub[t][i] <= 1.002 * m && (tb < 0 || per_sub[t][i] < per_sub[tb][i])) tb = t;
             if (tb >= 0) {
                 best_k = counts[i];
                 best_tile = tiles[tb];
