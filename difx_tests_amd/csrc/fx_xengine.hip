@@ -146,13 +146,18 @@ template <> struct XChunk<FX_PREC_F8>  { static constexpr int KC = 8; };   // 64
 // work of a 32x32 block for the same K.  Q3 (FP32, one block of 17 to 32
 // signals): its quarters (0, 0), (0, 1) and (1, 1) on v_mfma_f32_16x16x4f32,
 // three quarters of the work; quarter (1, 0) holds the conjugates of (0, 1),
-// and no visibility is read from it (the export reads i <= j).
+// and no visibility is read from it (the export reads i <= j).  H8 (FP32, at
+// most 8 signals): a 16x16 MFMA over two groups at once, rows 0-7 of its
+// operand from one group and rows 8-15 from the next, so its diagonal 8x8
+// quarters are the two groups' products (summed at the end) and the others
+// products across FFTs, never stored: half the MFMAs and staged rows of H16.
 // NTL also means one panel pair per channel (every signal in one diagonal
 // super-tile): one panel is staged, each thread issues one branch-free load
 // per chunk, and the loads run two chunks ahead in two register sets (the
 // waits before the LDS stores then cover only the older chunk; a conditional
 // load would make the compiler wait for every load in flight).
-template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false, bool H16 = false, bool Q3 = false>
+template <int PREC, int SB, int NSUB, int NW = WAVES, bool NTL = false, bool H16 = false, bool Q3 = false,
+          bool H8 = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 2))) void fx_xengine_kernel(XArgs a)
 {
     constexpr int KC = XChunk<PREC>::KC;
@@ -160,9 +165,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     // rows staged per group and panel, and panels staged: H16 stages only the
     // 16 rows its quarter block reads, of its one (diagonal) panel; NTL the
     // one (diagonal) panel of its grid
-    constexpr bool QB = H16 || Q3;                       // 16x16 quarters
+    constexpr bool QB = H16 || Q3 || H8;                 // 16x16 quarters
     constexpr int NQ = Q3 ? 3 : 1;
-    constexpr int PWL = H16 ? FX_BLK / 2 : PW;
+    constexpr int PWL = H16 ? FX_BLK / 2 : H8 ? FX_BLK / 4 : PW;
     constexpr int NPL = (QB || NTL) ? 1 : 2;
     constexpr int PF = NTL ? 2 : 1;                      // chunks in flight per thread
     static_assert(PF == 1 || NSUB == 1, "two chunks in flight: one chunk per step");
@@ -357,7 +362,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
         }
     };
 
-    static_assert(!QB || (PREC == FX_PREC_F32 && SB == 1 && !(H16 && Q3)), "16x16 blocks: FP32, one block per pair");
+    static_assert(!QB || (PREC == FX_PREC_F32 && SB == 1 && (int)H16 + (int)Q3 + (int)H8 == 1),
+                  "16x16 blocks: FP32, one block per pair");
     f32x16 accr[NBW], acci[NBW];
 #pragma unroll
     for (int t = 0; t < NBW; t++) {
@@ -453,6 +459,22 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
                 const float cv = __int_as_float(__float_as_int(cw) ^ (k4 >= 2 ? (int)0x80000000 : 0));
                 acr16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, av, acr16[0], 0, 0, 0);
                 aci16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, av, aci16[0], 0, 0, 0);
+            }
+        } else if constexpr (PREC == FX_PREC_F32 && H8) {
+            // item q of chunk sub: groups 2q (operand rows 0-7) and 2q + 1
+            // (rows 8-15); the items of a step are dealt to the waves in turn
+            static_assert(KC == NW && KC % 2 == 0, "pairs of groups");
+            if (okm & 1u) {
+#pragma unroll
+                for (int q = 0; q < KC / 2; q++) {
+                    if (((sub * (KC / 2) + q) & (NW - 1)) != wvu) continue;
+                    const float* rp = reinterpret_cast<const float*>(&L[sub][0][2 * q + (r16 >> 3)][r16 & 7]);
+                    const float av = rp[k4];
+                    const float cw = rp[(k4 + 2) & 3];
+                    const float cv = __int_as_float(__float_as_int(cw) ^ (k4 >= 2 ? (int)0x80000000 : 0));
+                    acr16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, av, acr16[0], 0, 0, 0);
+                    aci16[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv, av, aci16[0], 0, 0, 0);
+                }
             }
         } else if constexpr (PREC == FX_PREC_F32 && Q3) {
             // as H16, for rows r16 (quarter row 0) and 16 + r16 (quarter row
@@ -630,8 +652,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
             __syncthreads();
         }
         if (wave != 0) okm = 0;
+        if constexpr (H8) {
+            // V[m][n] = D[m][n] + D[m + 8][n + 8] for m, n < 8: lane l holds
+            // D[4 (l / 16) + v][l % 16], so D[m + 8][n + 8] is in lane l + 40;
+            // every other entry is set to zero (added to the accumulators, it
+            // leaves them as they are).  Wave 0 alone, through the staging LDS.
+            if (wave == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    red[v * 64 + lane] = acr16[0][v];
+                    red[(4 + v) * 64 + lane] = aci16[0][v];
+                }
+                const bool keep = (lane >> 4) < 2 && (lane & 15) < 8;
+                const int src = (lane + 40) & 63;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    acr16[0][v] = keep ? acr16[0][v] + red[v * 64 + src] : 0.0f;
+                    aci16[0][v] = keep ? aci16[0][v] + red[(4 + v) * 64 + src] : 0.0f;
+                }
+            }
+        }
     }
-    if constexpr (H16) {
+    if constexpr (H16 || H8) {
         if (okm & 1u) store_block16(a, c, lane, acr16[0], aci16[0]);
         return;
     }
@@ -824,6 +866,15 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
     case 2: fx_xengine_kernel<PREC, 2, 1><<<grid, block, 0, s>>>(a); break;
     default:
         if constexpr (PREC == FX_PREC_F32) {
+            if (a.p.S <= FX_BLK / 4) {                    // at most 8 signals: two groups per 16x16 MFMA
+                // (at least two chunks per step: the partial-sum reduction
+                // needs the 2 KB that they stage)
+                if ((long long)a.p.nchan * npairs <= kSmallGrid)
+                    fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, false, false, true><<<grid, block, 0, s>>>(a);
+                else
+                    fx_xengine_kernel<PREC, 1, 2, WAVES, false, false, false, true><<<grid, block, 0, s>>>(a);
+                break;
+            }
             if (a.p.S <= FX_BLK / 2) {                    // at most 16 signals: 16x16 MFMA blocks
                 if ((long long)a.p.nchan * npairs <= kSmallGrid)
                     fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, true><<<grid, block, 0, s>>>(a);

@@ -111,10 +111,12 @@ def test_graph_replays_against_eager():
 # over K loops of 1 to 5 chunks: 16 FFTs per FP16 chunk, 64 per FP8 chunk);
 # 48 and 64 antennas: 3 and 4 blocks, one panel pair per channel at tile 4
 # (the FP16 kernel that stages one panel and keeps two K chunks in flight),
-# over K loops of 1, 3, 4 and 5 chunks of 16 FFTs; 8, 12 and 16 antennas FP32:
-# one block per channel, at tile 1 formed as 16x16 quarters (one quarter at
-# up to 16 signals, three above), at tiles 2 and 4 as a 32x32 block
-@pytest.mark.parametrize("prec,nant,nf", [("f32", 8, 16), ("f32", 12, 16), ("f32", 16, 48), ("f32", 16, 4),
+# over K loops of 1, 3, 4 and 5 chunks of 16 FFTs; 1 to 16 antennas FP32:
+# one block per channel, at tile 1 formed on 16x16 MFMAs (two groups per
+# MFMA at up to 8 signals, one quarter at up to 16, three above), at tiles 2
+# and 4 as a 32x32 block
+@pytest.mark.parametrize("prec,nant,nf", [("f32", 1, 16), ("f32", 3, 48), ("f32", 4, 16), ("f32", 4, 40),
+                                          ("f32", 8, 16), ("f32", 12, 16), ("f32", 16, 48), ("f32", 16, 4),
                                           ("f32", 70, 4), ("f16", 70, 4), ("f32v", 70, 4), ("f16", 70, 48),
                                           ("f16", 70, 80), ("f8", 70, 64), ("f8", 70, 128), ("f8", 70, 192),
                                           ("f16", 48, 4), ("f16", 48, 48), ("f16", 64, 64), ("f16", 64, 80)])
@@ -139,7 +141,8 @@ def test_xtile_does_not_change_results(prec, nant, nf):
     cross1, cc1, autos1, ac1 = out[1]
     cross2, cc2, autos2, ac2 = out[2]
     assert np.array_equal(cc1, cc2) and np.array_equal(ac1, ac2)
-    assert np.abs(cross1 - cross2).max() <= 1e-6 * np.abs(cross2).max()
+    if cross2.size:                          # (one antenna: autocorrelations only)
+        assert np.abs(cross1 - cross2).max() <= 1e-6 * np.abs(cross2).max()
     assert np.abs(autos1 - autos2).max() <= 1e-6 * np.abs(autos2).max()
 
 
