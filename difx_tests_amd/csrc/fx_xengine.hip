@@ -162,9 +162,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
 {
     constexpr int KC = XChunk<PREC>::KC;
     constexpr int PW = SB * FX_BLK;                      // signals per panel
-    // rows staged per group and panel, and panels staged: H16 stages only the
-    // 16 rows its quarter block reads, of its one (diagonal) panel; NTL the
-    // one (diagonal) panel of its grid
+    // rows staged per group and panel, and panels staged: the 16x16 kernels
+    // stage only the rows they read (16 for H16, 8 for H8, 32 for Q3) of their
+    // one (diagonal) panel; NTL the one (diagonal) panel of its grid
     constexpr bool QB = H16 || Q3 || H8;                 // 16x16 quarters
     constexpr int NQ = Q3 ? 3 : 1;
     constexpr int PWL = H16 ? FX_BLK / 2 : H8 ? FX_BLK / 4 : PW;
@@ -174,8 +174,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8
     constexpr int NELEM = NPL * PWL * KC;                // 16-byte rows per chunk
     constexpr int NLD = (NSUB * NELEM + 64 * NW - 1) / (64 * NW);
     constexpr bool PART = (NSUB * NELEM) % (64 * NW) != 0;   // threads past the step's rows load nothing
-    // [stage][chunk][panel][group][signal], the panels and rows staged; H16's
-    // partial-sum reduction reuses it
+    // [stage][chunk][panel][group][signal], the panels and rows staged; the
+    // K-split kernels' partial-sum reduction reuses it
     // (direct-to-LDS kernels keep their second stage in an array of its own,
     // so that the compiler sees that loads into one stage do not alias reads
     // of the other)
