@@ -1086,11 +1086,12 @@ void fx_fstage_kernel(FStageArgs args)
     }
 
     // Twiddle factors whose exponents depend only on the thread, loaded once
-    // where the register budget allows (three waves per SIMD, and at F = 4096
-    // with four, where FP32 and FP16 have the room: 120 and 124 registers, 3%
-    // and 5% fewer VALU instructions): the third pass's of a four-pass
-    // Stockham plan and the last pass's.
-    constexpr bool TWREG = (fstage_wpe<LG, V, PREC, CPLX>() == 3 || LG == 12) && !XL && !DIF && !CPLX;
+    // where the register budget allows (three waves per SIMD, and with four at
+    // F = 4096 and, except FP8, F = 16384: 103-127 registers, 1-5% fewer VALU
+    // instructions; the FP8 16384-point kernels spill with them): the third
+    // pass's of a four-pass Stockham plan and the last pass's.
+    constexpr bool TWREG = (fstage_wpe<LG, V, PREC, CPLX>() == 3 || LG == 12 || (LG == 14 && PREC != FX_PREC_F8)) &&
+                           !XL && !DIF && !CPLX;
     constexpr int PER2 = (F / R2) / NT > 0 ? (F / R2) / NT : 1;
     [[maybe_unused]] TwFac<R2> fac2[TWREG && Gm::NP == 4 ? PER2 : 1];
     [[maybe_unused]] TwFac<RL> facl[TWREG ? PERL : 1];
