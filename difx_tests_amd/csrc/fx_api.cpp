@@ -80,6 +80,7 @@ struct fx_context {
     float2* d_autos;
     float2* d_tap;
     float* d_ltap;      // decoded levels of the tap (FX_TAP_LEVELS)
+    void* d_stage;      // F = 32768: the F-stage's row staging (fx_fstage_stage_bytes)
     int tap;            // FX_TAP_* bits
     int last_nffts;     // entries in the delay table of the last fx_fstage
     // pinned ring of {origin, first FFT index, list flag} blocks copied to
@@ -155,7 +156,7 @@ static void free_ctx(fx_context* c)
     if (c->cstream) (void)hipStreamDestroy(c->cstream);
     void* bufs[] = {c->d_data, c->d_data1, c->d_model, c->d_iter, c->d_dtab, c->d_valid, c->d_rtab, c->d_tw, c->d_subchf, c->d_stepchf,
                     c->d_xin, c->d_vacc, c->d_counts, c->d_scale, c->d_scale_inv,
-                    c->d_cross, c->d_autos, c->d_tap, c->d_ltap};
+                    c->d_cross, c->d_autos, c->d_tap, c->d_ltap, c->d_stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto* v : {&c->pool, &c->pending})
@@ -280,6 +281,7 @@ extern "C" int fx_create(const fx_params* prm, fx_context** out)
     ALLOC(c->d_dtab, sizeof(FxDelay) * (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_valid, (size_t)c->max_ffts * p.nant);
     ALLOC(c->d_rtab, sizeof(float2) * (size_t)c->max_ffts * p.nant * fx_rtab_len(p));
+    if (const size_t sb = fx_fstage_stage_bytes(p.lg, c->fprec, p.nant, c->nfg)) ALLOC(c->d_stage, sb);
     ALLOC(c->d_tw, sizeof(float2) * p.F);
     ALLOC(c->d_subchf, sizeof(float) * p.stride);
     ALLOC(c->d_stepchf, sizeof(float) * p.nfstep);
@@ -531,6 +533,7 @@ static int fstage_enqueue(fx_context* c, const FStageCall& f, hipStream_t s)
     a.scale_inv = c->d_scale_inv;
     a.tap = c->tap ? c->d_tap : nullptr;
     a.ltap = (c->tap & FX_TAP_LEVELS) && !p.iscomplex ? c->d_ltap : nullptr;
+    a.stage = c->d_stage;
     a.plan = c->fplan;
     TIMED(TM_FSTAGE, s, launch_fstage(a, c->fprec, s));
     if (ingest) HIPCHK(hipEventRecord(c->slot_free[f.slot], s));           // the slot may be overwritten

@@ -1453,29 +1453,63 @@ void fx_fstage_kernel(FStageArgs args)
 // the LDS, so it is formed from its even- and odd-sample halves,
 //   X[c] = E[c] + W^c O[c],   c < F/2 (exactly the channels kept),
 // each half a 16384-point transform on the F = 16384 passes.  E stays in
-// registers while O is formed.  Both halves read the packed bytes straight
-// from global memory (one time slot each, so the byte index is contiguous
-// in the thread index); outputs go to the corner-turned rows one FFT slot at
-// a time.
+// registers while O is formed.  The FFT's packed bytes (F/2 + 1 of them) are
+// staged in the LDS once, with coalesced word loads, and each half reads its
+// time slot from there (the byte index is contiguous in the thread index).
+// A block's G FFTs share each 16-byte output row, and one FFT's outputs do
+// not fit in registers beside the next FFT's work, so FFTs 0 .. G-2 park
+// their encoded outputs in the block's staging area in thread order (whole
+// cache lines) and FFT G-1 reads them back and writes every row whole: one
+// 16-byte store per row instead of G scattered 2- to 8-byte pieces (the L2
+// takes one request per piece).
 
 static constexpr int kEoLG = 15;
 
+// One FFT's output value of one row as the row stores it: FP32 (re, im),
+// FP16 half2, FP8 two e4m3 bytes (re low).
+template <int PREC> struct SlotVal;
+template <> struct SlotVal<FX_PREC_F32> { using T = float2; };
+template <> struct SlotVal<FX_PREC_F16> { using T = uint32_t; };
+template <> struct SlotVal<FX_PREC_F8>  { using T = uint16_t; };
+
 template <int PREC>
-__device__ __forceinline__ void store_slot(const FStageArgs& args, long long row, int gi, float2 y, float si)
+__device__ __forceinline__ typename SlotVal<PREC>::T encode_slot(float2 y, float si)
 {
     if constexpr (PREC == FX_PREC_F32) {
-        float* dst = reinterpret_cast<float*>(args.out) + row * 4;      // (re0, re1, im0, im1)
-        dst[gi] = y.x;
-        dst[2 + gi] = y.y;
+        return y;
     } else if constexpr (PREC == FX_PREC_F16) {
         const __half2 h = __floats2half2_rn(y.x, y.y);
-        reinterpret_cast<uint32_t*>(args.out)[row * 4 + gi] = *reinterpret_cast<const uint32_t*>(&h);
+        return *reinterpret_cast<const uint32_t*>(&h);
     } else {
         const float a = fminf(fmaxf(y.x * si, -448.0f), 448.0f);
         const float b = fminf(fmaxf(y.y * si, -448.0f), 448.0f);
-        const uint32_t w = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
-        reinterpret_cast<uint16_t*>(args.out)[row * 8 + gi] = (uint16_t)(w & 0xffffu);
+        return (uint16_t)((uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xffffu);
     }
+}
+
+// Row of G FFTs from the first G-1 (v[0 .. G-2]) and the last (y)
+template <int PREC, int G>
+__device__ __forceinline__ uint4 make_row(const typename SlotVal<PREC>::T (&v)[G - 1], typename SlotVal<PREC>::T y)
+{
+    if constexpr (PREC == FX_PREC_F32) {
+        static_assert(G == 2, "FP32 rows: (re0, re1, im0, im1)");
+        return make_uint4(__float_as_uint(v[0].x), __float_as_uint(y.x), __float_as_uint(v[0].y), __float_as_uint(y.y));
+    } else if constexpr (PREC == FX_PREC_F16) {
+        static_assert(G == 4, "FP16 rows: four half2");
+        return make_uint4(v[0], v[1], v[2], y);
+    } else {
+        static_assert(G == 8, "FP8 rows: eight (re, im) byte pairs");
+        return make_uint4(v[0] | (uint32_t)v[1] << 16, v[2] | (uint32_t)v[3] << 16, v[4] | (uint32_t)v[5] << 16,
+                          v[6] | (uint32_t)y << 16);
+    }
+}
+
+size_t fx_fstage_stage_bytes(int lg, int prec, int nant, int ngroups)
+{
+    if (lg != kEoLG) return 0;
+    const size_t vb = prec == FX_PREC_F32 ? sizeof(float2) : prec == FX_PREC_F16 ? sizeof(uint32_t) : sizeof(uint16_t);
+    const int G = fx_fstage_group_size(prec);
+    return (size_t)2 * ngroups * nant * (G - 1) * ((size_t)1 << (kEoLG - 1)) * vb;
 }
 
 template <int V, int PREC, bool TAP>
@@ -1496,11 +1530,14 @@ void fx_fstage_eo_kernel(FStageArgs args)
     constexpr int NTW1 = R0 * R1;
 
     __shared__ __align__(16) float2 buf[FP];
-    // level of this pol at [byte][time slot]: the bytes come straight from
-    // global memory here, so they are not recoded to code pairs
+    // level of this pol at [byte][time slot] (the window keeps raw bytes)
     __shared__ __align__(16) float lut[512];
     __shared__ __align__(16) float2 tab[NTAB];
     __shared__ float2 tw1[R0 * (R1 + 1)];
+    // the FFT's window: F samples span at most F/2 + 4 bytes from its aligned
+    // start, F/8 + 1 words (141 + 16 KB of LDS in all)
+    constexpr int NWIN = F / 8 + 1;
+    __shared__ __align__(16) uint32_t win[NWIN];
     const float2* rsub = tab;
     const float2* rstep = tab + SUB;
     const float2* frsub = tab + SUB + NSTEP;
@@ -1529,93 +1566,165 @@ void fx_fstage_eo_kernel(FStageArgs args)
     }
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
 
+    // The next FFT's delay entry, rotator table and window are loaded into
+    // registers before this FFT's output stores are issued (the stores are
+    // scattered 2- to 4-byte writes, and loads issued after them wait for
+    // them); the delay entry (offset, valid) one FFT earlier still.
+    static_assert(NTAB <= NT, "one table entry per thread");
+    constexpr int WPT = (NWIN + NT - 1) / NT;
+    struct Prefetch {
+        int valid, offset;
+        float2 t;
+        uint32_t w[WPT];
+    };
+    auto dload = [&](int gi) { return load_delay_ov(args.delays + (long long)(grp * G + gi) * p.nant + ant); };
+    auto fetch = [&](int gi, Prefetch& f, int2 d) {
+        f.offset = d.x;
+        f.valid = d.y;
+        if (!f.valid) return;                                  // uniform across the block
+        const float2* tb = args.rtab + ((long long)(grp * G + gi) * p.nant + ant) * (NTAB + NSTEP);
+        const int bb = (f.offset >> 1) & ~3;
+        const int nd = ((((f.offset + F - 1) >> 1) - bb) >> 2) + 1;
+        const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src + bb);
+        f.t = threadIdx.x < NTAB ? tb[threadIdx.x] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int w = 0; w < WPT; w++) {
+            const int k = threadIdx.x + w * NT;
+            f.w[w] = k < nd ? s32[k] : 0u;
+        }
+    };
+    Prefetch cur;
+    fetch(0, cur, dload(0));
+    int2 dnext = dload(G > 1 ? 1 : 0);
+
 #pragma unroll 1
     for (int gi = 0; gi < G; gi++) {
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid >= 0 && tid < NT);
         const int fft = grp * G + gi;
-        const FxDelay d = args.delays[(long long)fft * p.nant + ant];
-        __syncthreads();
-        if (!d.valid) {
+        __syncthreads();                                       // previous FFT done with the LDS
+        float2 ev[PERL][RL];                                   // E[c], then the output y[c]; c = tid + m NT + r PL
+        if (!cur.valid) {
+            if (gi + 1 < G) {
+                fetch(gi + 1, cur, dnext);
+                if (gi + 2 < G) dnext = dload(gi + 2);
+            }
 #pragma unroll
             for (int m = 0; m < PERL; m++)
 #pragma unroll
-                for (int r = 0; r < RL; r++) {
-                    const int c = tid + m * NT + r * PL;
-                    store_slot<PREC>(args, xrow0 + (long long)c * cpitch, gi,
-                                     make_float2(0.0f, 0.0f), 0.0f);
-                    if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = make_float2(0.0f, 0.0f);
-                }
-            continue;
-        }
-        {
-            const float2* tb = args.rtab + ((long long)fft * p.nant + ant) * (NTAB + NSTEP);
-            for (int e = tid; e < NTAB; e += NT) tab[e] = tb[e];
-        }
-        __syncthreads();
-
-        const int o = d.offset;
-        float2 ev[PERL][RL];                                   // E[c], c = tid + m NT + r PL
-#pragma unroll 1
-        for (int h = 0; h < 2; h++) {
-            asm volatile("" : "+v"(tid));
-            __builtin_assume(tid >= 0 && tid < NT);
-            // ---- pass 0 of half h: samples n = 2 (i + r T0) + h, one time slot
-            {
-                const int i = tid;
-                const int n0 = 2 * i + h;
-                const float2 rs = rsub[n0 & (SUB - 1)];
-                const float2* sp = rstep + n0 / SUB;
-                const float* lt = lut + ((o + h) & 1);
-                const uint8_t* bp = src + ((o + h) >> 1) + i;
-                float2 x[R0];
+                for (int r = 0; r < RL; r++) ev[m][r] = make_float2(0.0f, 0.0f);
+        } else {
+            const int o = cur.offset;
+            const int bb = (o >> 1) & ~3;
+            const int nd = ((((o + F - 1) >> 1) - bb) >> 2) + 1;
+            if (tid < NTAB) tab[tid] = cur.t;
 #pragma unroll
-                for (int r = 0; r < R0; r++) {
-                    const float l = lt[2 * bp[r * T0]];
-                    if (TAP && args.ltap)
-                        args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + 2 * (i + r * T0) + h] = l;
-                    x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (2 * T0 / SUB)]);
-                }
-                dft<R0>(x);
-                float2* wr = buf + Gh::pad(i * R0);
-#pragma unroll
-                for (int r = 0; r < R0; r++) wr[r] = x[r];
+            for (int w = 0; w < WPT; w++) {
+                const int k = tid + w * NT;
+                if (k < nd) win[k] = cur.w[w];
             }
             __syncthreads();
-            mid_pass<Gh, R1, R0, true, 2>(buf, tw1, args.tw, tid);
-            mid_pass<Gh, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
-            // ---- last pass of half h, every bin
+
+#pragma unroll 1
+            for (int h = 0; h < 2; h++) {
+                asm volatile("" : "+v"(tid));
+                __builtin_assume(tid >= 0 && tid < NT);
+                // ---- pass 0 of half h: samples n = 2 (i + r T0) + h, one time slot
+                {
+                    const int i = tid;
+                    const int n0 = 2 * i + h;
+                    const float2 rs = rsub[n0 & (SUB - 1)];
+                    const float2* sp = rstep + n0 / SUB;
+                    const float* lt = lut + ((o + h) & 1);
+                    const uint8_t* bp = reinterpret_cast<const uint8_t*>(win) + (((o + h) >> 1) - bb) + i;
+                    float2 x[R0];
 #pragma unroll
-            for (int m = 0; m < PERL; m++) {
-                const int i = tid + m * NT;
-                float2 x[RL];
-                const float2* rd = buf + Gh::pad(i);
+                    for (int r = 0; r < R0; r++) {
+                        const float l = lt[2 * bp[r * T0]];
+                        if (TAP && args.ltap)
+                            args.ltap[(((long long)fft * p.nant + ant) * 2 + pol) * F + 2 * (i + r * T0) + h] = l;
+                        x[r] = cmul(make_float2(l * rs.x, l * rs.y), sp[r * (2 * T0 / SUB)]);
+                    }
+                    dft<R0>(x);
+                    float2* wr = buf + Gh::pad(i * R0);
 #pragma unroll
-                for (int r = 0; r < RL; r++) x[r] = rd[Gh::pad(r * TL)];
-                twiddle_pow<RL>(x, args.tw, 2 * i);
-                dft<RL>(x);
-                if (h == 0) {
+                    for (int r = 0; r < R0; r++) wr[r] = x[r];
+                }
+                __syncthreads();
+                mid_pass<Gh, R1, R0, true, 2>(buf, tw1, args.tw, tid);
+                mid_pass<Gh, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
+                // ---- last pass of half h, every bin, into registers.  One
+                // table entry per butterfly, all loaded ahead: W^(2i) for the
+                // first half, W^i for the second (its square is W^(2i)); the
+                // twiddles W^(2ir) are products of those, at most three deep.
+                float2 lw[PERL];
 #pragma unroll
-                    for (int r = 0; r < RL; r++) ev[m][r] = x[r];
-                } else {
-                    // W^c = W^i * exp(-2 pi i r PL / F); PL / F = 1/8
-                    const float2 wi = args.tw[i];
+                for (int m = 0; m < PERL; m++) lw[m] = args.tw[(2 - h) * (tid + m * NT)];
 #pragma unroll
-                    for (int r = 0; r < RL; r++) {
-                        const int c = i + r * PL;
-                        const float2 ov = twc(cmul(x[r], wi), r * (32 * PL / F));
-                        const float2 X = cadd(ev[m][r], ov);
-                        const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
-                        const float2 y = cmul(fr, X);
-                        float si = 1.0f;
-                        if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
-                        store_slot<PREC>(args, xrow0 + (long long)c * cpitch, gi, y, si);
-                        if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = y;
+                for (int m = 0; m < PERL; m++) {
+                    const int i = tid + m * NT;
+                    float2 x[RL];
+                    const float2* rd = buf + Gh::pad(i);
+#pragma unroll
+                    for (int r = 0; r < RL; r++) x[r] = rd[Gh::pad(r * TL)];
+                    {
+                        float2 w[RL];
+                        w[1] = h == 0 ? lw[m] : cmul(lw[m], lw[m]);
+#pragma unroll
+                        for (int r = 2; r < RL; r++) {
+                            const int hb = 1 << clog2(r);                  // largest power of 2 <= r
+                            w[r] = r == hb ? cmul(w[r / 2], w[r / 2]) : cmul(w[hb], w[r - hb]);
+                        }
+#pragma unroll
+                        for (int r = 1; r < RL; r++) x[r] = cmul(x[r], w[r]);
+                    }
+                    dft<RL>(x);
+                    if (h == 0) {
+#pragma unroll
+                        for (int r = 0; r < RL; r++) ev[m][r] = x[r];
+                    } else {
+                        // W^c = W^i * exp(-2 pi i r PL / F); PL / F = 1/8
+                        const float2 wi = lw[m];
+#pragma unroll
+                        for (int r = 0; r < RL; r++) {
+                            const int c = i + r * PL;
+                            const float2 ov = twc(cmul(x[r], wi), r * (32 * PL / F));
+                            const float2 X = cadd(ev[m][r], ov);
+                            const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
+                            ev[m][r] = cmul(fr, X);
+                        }
                     }
                 }
+                if (h == 0) __syncthreads();                   // buf is rewritten by the second half
             }
-            __syncthreads();                                   // buf is rewritten by the next half / FFT
+            if (gi + 1 < G) {
+                fetch(gi + 1, cur, dnext);
+                if (gi + 2 < G) dnext = dload(gi + 2);
+            }
         }
+        // ---- this FFT's slot of every output row: staged (FFTs 0 .. G-2, at
+        // [FFT][m][r][thread] of the block's area) or, with the staged slots
+        // read back, the whole row (FFT G-1)
+        using SV = typename SlotVal<PREC>::T;
+        SV* const stg = reinterpret_cast<SV*>(args.stage) + (size_t)blockIdx.x * (G - 1) * NCH;
+#pragma unroll
+        for (int m = 0; m < PERL; m++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                const int c = tid + m * NT + r * PL;
+                float si = 1.0f;
+                if (PREC == FX_PREC_F8) si = args.scale_inv[(long long)sig * NCH + c];
+                const SV v = encode_slot<PREC>(ev[m][r], si);
+                if (gi < G - 1) {
+                    stg[((gi * PERL + m) * RL + r) * NT + tid] = v;
+                } else {
+                    SV pv[G - 1];
+#pragma unroll
+                    for (int g = 0; g < G - 1; g++) pv[g] = stg[((g * PERL + m) * RL + r) * NT + tid];
+                    reinterpret_cast<uint4*>(args.out)[xrow0 + (long long)c * cpitch] = make_row<PREC, G>(pv, v);
+                }
+                if (TAP) args.tap[(((long long)fft * p.nant + ant) * 2 + pol) * NCH + c] = ev[m][r];
+            }
     }
 }
 
@@ -1636,6 +1745,7 @@ static hipError_t launch_eo(const FStageArgs& a, int prec, hipStream_t s)
 {
     const int nblocks = 2 * a.ngroups * a.p.nant;
     if (nblocks <= 0) return hipSuccess;
+    if (!a.stage) return hipErrorInvalidValue;
     const dim3 grid(nblocks);
     const bool tap = a.tap != nullptr || a.ltap != nullptr;
     switch (prec) {

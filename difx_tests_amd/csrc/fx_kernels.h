@@ -29,6 +29,7 @@ struct FStageArgs {
     float2* tap;                  // optional FP32 spectra [nffts][nant][2][nchan]
     float* ltap;                  // optional decoded 2-bit levels [nffts][nant][2][F] of the
                                   // valid FFTs (real input; only with tap)
+    void* stage;                  // F = 32768: row staging, fx_fstage_stage_bytes(lg, prec, nant, ngroups) bytes
     int plan;                     // radix plan, 0 .. fstage_plans(lg) - 1
     int tile_ant;                 // antennas per work tile (set by launch_fstage)
 };
@@ -72,6 +73,9 @@ constexpr int kDelayArgIt0 = 4, kDelayArgOrigin = 9, kDelayNArgs = 16;
 // that q, or 0 when the plan reads them from the LDS.
 int fstage_step_q(int lg, int plan, int iscomplex);
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
+// Device bytes of FStageArgs::stage for a launch of ngroups groups (0 where
+// the F-stage stages nothing, F < 32768)
+size_t fx_fstage_stage_bytes(int lg, int prec, int nant, int ngroups);
 bool fstage_supported(int lg);
 bool fstage_supported_complex(int lg);   // complex 8-bit input, F = nchan
 int fstage_plans(int lg);

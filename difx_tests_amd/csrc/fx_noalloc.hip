@@ -136,10 +136,12 @@ extern "C" int fx_process_noalloc(const fx_params* prm, const uint8_t* data, lon
                          up256(sizeof(float) * p.nfstep) + up256(vacc_bytes) +
                          up256(sizeof(int) * (size_t)p.nant * p.nant) +
                          (fprec == FX_PREC_F8 ? 2 * up256(sizeof(float) * (size_t)p.nchan * p.Spad) : 0);
-    // per chunk of K FFTs: delay entries, validity bytes, rotator tables, X input
+    // per chunk of K FFTs: delay entries, validity bytes, rotator tables, X
+    // input, and at F = 32768 the F-stage's row staging
     auto per_chunk = [&](long long K) {
         return up256(sizeof(FxDelay) * K * p.nant) + up256((size_t)K * p.nant) +
-               up256(sizeof(float2) * K * p.nant * fx_rtab_len(p)) + up256((size_t)(K / G / kc) * p.nchan * fx_xin_pitch(kc, p.Spad) * 16);
+               up256(sizeof(float2) * K * p.nant * fx_rtab_len(p)) + up256((size_t)(K / G / kc) * p.nchan * fx_xin_pitch(kc, p.Spad) * 16) +
+               up256(fx_fstage_stage_bytes(p.lg, fprec, p.nant, (int)(K / G)));
     };
     long long K = npad;
     while (K > gran && fixed + per_chunk(K) > kScratchBytes) K -= gran;
@@ -174,6 +176,7 @@ extern "C" int fx_process_noalloc(const fx_params* prm, const uint8_t* data, lon
     float2* rtab = reinterpret_cast<float2*>(take(sizeof(float2) * K * p.nant * fx_rtab_len(p)));
     const size_t xin_bytes = (size_t)(K / G / kc) * p.nchan * fx_xin_pitch(kc, p.Spad) * 16;
     void* xin = take(xin_bytes);
+    void* stage = take(fx_fstage_stage_bytes(p.lg, fprec, p.nant, (int)(K / G)));
 
     hipStream_t s = static_cast<hipStream_t>(stream);
     // The array keeps the previous call's contents: clear what accumulates and
@@ -216,6 +219,7 @@ extern "C" int fx_process_noalloc(const fx_params* prm, const uint8_t* data, lon
         a.scale_inv = scale_inv;
         a.tap = nullptr;
         a.ltap = nullptr;
+        a.stage = stage;
         a.plan = 0;
         CHK(launch_fstage(a, fprec, s));
         XArgs x;

@@ -343,11 +343,12 @@ def test_vis_lowprec_every_plan_vs_fp64_oracle(nchan, nf, plan, prec):
 
 
 @pytest.mark.parametrize("prec", ["f16", "f8"])
-@pytest.mark.parametrize("nchan", [1024, 4096])
+@pytest.mark.parametrize("nchan", [1024, 4096, 16384])
 def test_vis_lowprec_invalid_ffts_vs_fp64_oracle(nchan, prec):
     # net delays that make whole FFTs of an antenna invalid (the first, the
     # last, clamped edges): the F-stage zeroes those slots of its register
-    # chain (FP8: one half of a word, FFTs in pairs) and the counts skip them
+    # chain (FP8: one half of a word, FFTs in pairs; at 16384 channels the
+    # staged slots of a row) and the counts skip them
     nant, nf = 5, 32
     p, data, delays, fileoff, it = _case(nchan, nant, nf, 600 + nchan, spread=2.4, data="correlated")
     with _engine(p, nant, nf, prec=prec, data=data) as e:
