@@ -70,6 +70,10 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261205)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three plain 512 MiB device copies (calibrates the HBM traffic counters)")
+    ap.add_argument("--subbands-only", action="store_true",
+                    help="only run the eight-subband batch measurement (autotuned unless --no-autotune) and print it")
+    ap.add_argument("--subband-modes", default="launch,graph",
+                    help="modes of the subband batch measurement: launch, graph or both (comma-separated)")
     return ap.parse_args()
 
 
@@ -178,7 +182,7 @@ def run_pipeline(w, prec, data, delays, fileoff, nf, steps, warmup, xbatch, tune
         e.close()
 
 
-def subband_batch(steps, warmup, seed, prec="f32", tune=True):
+def subband_batch(steps, warmup, seed, prec="f32", tune=True, modes=("launch", "graph")):
     """Eight uneven subbands of a 16-antenna array, one 10 ms subint per step:
     one hipGraph launch per subint (subbands as concurrent branches) against
     plain launches one subband after another.  Synthetic random 2-bit input."""
@@ -200,7 +204,7 @@ def subband_batch(steps, warmup, seed, prec="f32", tune=True):
            "xengine_prec": prec, "samples_per_subint": samples}
     try:
         with FxBatch(engines) as b:
-            for mode in ("launch", "graph"):
+            for mode in modes:
                 for _ in range(warmup):
                     b.process(iters, 1, mode)
                 b.sync()
@@ -210,7 +214,8 @@ def subband_batch(steps, warmup, seed, prec="f32", tune=True):
                 b.sync()
                 dt = (time.perf_counter() - t0) / steps
                 out[mode] = {"ms_per_subint": round(dt * 1e3, 4), "gsamples_s": round(samples / dt / 1e9, 3)}
-            out["graph_speedup"] = round(out["launch"]["ms_per_subint"] / out["graph"]["ms_per_subint"], 3)
+            if "launch" in out and "graph" in out:
+                out["graph_speedup"] = round(out["launch"]["ms_per_subint"] / out["graph"]["ms_per_subint"], 3)
             if tune:
                 choices = []
                 for e, w in zip(engines, subs):
@@ -391,6 +396,12 @@ def main():
     a = parse()
     if a.copy_probe:
         copy_probe()
+        return
+    if a.subbands_only:
+        modes = tuple(m for m in a.subband_modes.split(",") if m)
+        if not modes or any(m not in ("launch", "graph") for m in modes):
+            sys.exit("--subband-modes takes launch and/or graph")
+        print(json.dumps(subband_batch(max(5, a.steps), 2, a.seed, tune=not a.no_autotune, modes=modes)), flush=True)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
