@@ -41,7 +41,11 @@ typedef struct {
     double lo;                   /* LO frequency, Hz (< 2^31, kernel.c:39)         */
     double bw;                   /* bandwidth, Hz; sample time 1/(2*bw)            */
     int prec;                    /* FX_PREC_*                                      */
-    int max_ffts;                /* FFT slots held for the X-engine                */
+    int max_ffts;                /* FFT slots held for the X-engine (device memory
+                                    per slot and signal: the X input, nchan x 8, 4
+                                    or 2 bytes, and at nchan = 16384 the F-stage's
+                                    row staging, 64, 48 or 28 KB for FP32, FP16,
+                                    FP8)                                          */
     long long max_bytes_per_ant; /* packed input capacity per antenna              */
     int iscomplex;               /* 0: real 2-bit samples, F = 2*nchan (the
                                     reference's path); 1: complex 8-bit samples,
