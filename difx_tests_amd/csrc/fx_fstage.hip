@@ -1567,9 +1567,9 @@ void fx_fstage_eo_kernel(FStageArgs args)
     const uint8_t* src = args.data + (long long)ant * args.data_stride;
 
     // The next FFT's delay entry, rotator table and window are loaded into
-    // registers before this FFT's output stores are issued (the stores are
-    // scattered 2- to 4-byte writes, and loads issued after them wait for
-    // them); the delay entry (offset, valid) one FFT earlier still.
+    // registers before this FFT's output stores are issued (loads issued
+    // after them queue behind them); the delay entry (offset, valid) one FFT
+    // earlier still.
     static_assert(NTAB <= NT, "one table entry per thread");
     constexpr int WPT = (NWIN + NT - 1) / NT;
     struct Prefetch {
