@@ -43,7 +43,8 @@ def _rel(a, b):
     return (np.linalg.norm(a[live] - b[live], axis=-1) / np.linalg.norm(b[live], axis=-1)).max()
 
 
-@pytest.mark.parametrize("nchan,nant,prec", [(1024, 6, "f32"), (4096, 5, "f16"), (512, 9, "f8"), (2048, 4, "f32v")])
+@pytest.mark.parametrize("nchan,nant,prec", [(1024, 6, "f32"), (4096, 5, "f16"), (512, 9, "f8"), (2048, 4, "f32v"),
+                                             (16384, 3, "f32"), (16384, 2, "f8")])
 def test_batched_fstage_equals_per_subint_calls(nchan, nant, prec):
     F = 2 * nchan
     p = orc.plan(nchan, LO, BW)
