@@ -868,9 +868,13 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
         if constexpr (PREC == FX_PREC_F32) {
             if (a.p.S <= FX_BLK / 4) {                    // at most 8 signals: two groups per 16x16 MFMA
                 // (at least two chunks per step: the partial-sum reduction
-                // needs the 2 KB that they stage)
+                // needs the 2 KB that they stage).  In a small grid each
+                // workgroup's K loop is one memory latency per step, and a
+                // step of 16 chunks (8 KB) keeps four times the bytes of
+                // DEEP chunks in flight (4 antennas x 1024 channels: X-engine
+                // 0.86 -> 0.64 ms per 64 subints; 32 chunks measured no faster)
                 if ((long long)a.p.nchan * npairs <= kSmallGrid)
-                    fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, false, false, true><<<grid, block, 0, s>>>(a);
+                    fx_xengine_kernel<PREC, 1, 16, WAVES, false, false, false, true><<<grid, block, 0, s>>>(a);
                 else
                     fx_xengine_kernel<PREC, 1, 2, WAVES, false, false, false, true><<<grid, block, 0, s>>>(a);
                 break;
