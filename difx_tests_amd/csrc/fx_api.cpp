@@ -1065,8 +1065,8 @@ extern "C" int fx_set_option(fx_context* c, int option, int value)
         c->xsb = value;
         return FX_OK;
     case FX_OPT_FPLAN:
-        if (value < 0 || value >= fstage_plans(c->p.lg))
-            return fail(FX_EINVAL, "F-stage plan %d: %d plans at F=%d", value, fstage_plans(c->p.lg), c->p.F);
+        if (value < 0 || value >= fstage_plans(c->p.lg, c->p.iscomplex))
+            return fail(FX_EINVAL, "F-stage plan %d: %d plans at F=%d", value, fstage_plans(c->p.lg, c->p.iscomplex), c->p.F);
         c->fplan = value;
         return FX_OK;
     case FX_OPT_XBATCH:
@@ -1092,7 +1092,7 @@ extern "C" int fx_get_option(const fx_context* c, int option, int* value)
     switch (option) {
     case FX_OPT_XTILE: *value = c->xsb > 0 ? c->xsb : xengine_super_blocks(c->p.nblk); return FX_OK;
     case FX_OPT_FPLAN: *value = c->fplan; return FX_OK;
-    case FX_OPT_NPLANS: *value = fstage_plans(c->p.lg); return FX_OK;
+    case FX_OPT_NPLANS: *value = fstage_plans(c->p.lg, c->p.iscomplex); return FX_OK;
     case FX_OPT_XBATCH: *value = c->xbatch; return FX_OK;
     case FX_OPT_GRAPH: *value = c->use_graph; return FX_OK;
     case FX_OPT_FP8_AUTOSCALE: *value = c->autoscale; return FX_OK;
@@ -1339,7 +1339,7 @@ static void tune_apply_cached(fx_context* c)
     const std::string arch = device_arch();
     for (const TuneEntry& e : tune_read(path))
         if (e.arch == arch && e.nant == c->p.nant && e.nchan == c->p.nchan && e.prec == c->prec) {
-            if (e.plan >= 0 && e.plan < fstage_plans(c->p.lg)) c->fplan = e.plan;
+            if (e.plan >= 0 && e.plan < fstage_plans(c->p.lg, c->p.iscomplex)) c->fplan = e.plan;
             if (e.xtile == 1 || e.xtile == 2 || e.xtile == 4) c->xsb = e.xtile;
             if (e.xbatch >= 1) c->xbatch = e.xbatch;
         }
@@ -1423,7 +1423,7 @@ static int autotune_run(fx_context* c, int numffts, int reps, double* report, Tu
     // the best of the rounds kept (differences of a few percent must not
     // drown in the spread of single timings)
     constexpr int kRounds = 3;
-    const int nplan = fstage_plans(c->p.lg);
+    const int nplan = fstage_plans(c->p.lg, c->p.iscomplex);
     std::vector<double> fms(nplan, 1e30);
     for (int round = 0; rc == FX_OK && round < kRounds; round++)
         for (int v = 0; rc == FX_OK && v < nplan; v++) {

@@ -491,9 +491,13 @@ template <> struct FPlan<14, 1> { static constexpr int NP = 4, R0 = 16, R1 = 16,
 template <> struct FPlan<12, 2> { static constexpr int NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1, PADB = 4, KIND = 1; };
 template <> struct FPlan<13, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 2, PADB = 4, KIND = 2; };
 template <> struct FPlan<14, 2> { static constexpr int NP = 4, R0 = 16, R1 = 16, R2 = 16, R3 = 4, PADB = 4, KIND = 1; };
+// Three passes with 512 threads (real input only): one LDS exchange fewer than
+// the four-pass plans, and two waves per SIMD leave 256 registers a lane for
+// the radix-32 butterflies and the output chain.
+template <> struct FPlan<14, 3> { static constexpr int NP = 3, R0 = 16, R1 = 32, R2 = 32, R3 = 1, PADB = 5, KIND = 0; };
 
 // number of plans per FFT length
-__host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : (lg >= 12 && lg <= 14) ? 3 : 2; }
+__host__ __device__ constexpr int nplans(int lg) { return lg == 9 ? 1 : lg == 14 ? 4 : (lg >= 12 && lg <= 13) ? 3 : 2; }
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
@@ -526,8 +530,8 @@ template <int LG, int V = 0, bool CPLX = false> struct Geo {
     // so every offset from a butterfly's base address is a constant.
     static __host__ __device__ constexpr int pad(int n) { return n + (n >> PADB); }
     // waves per SIMD: at F = 8192 the LDS allows two 8-wave blocks per CU, at
-    // F = 16384 one 16-wave block (at F = 4096 see fstage_wpe)
-    static constexpr int WPE = LG >= 13 ? 4 : 3;
+    // F = 16384 one block of 16 (or, radix 32, 8) waves (at F = 4096 see fstage_wpe)
+    static constexpr int WPE = LG == 14 ? NT / 256 : LG >= 13 ? 4 : 3;
     static_assert(R0 * R1 * R2 * R3 == F, "plan must factor F");
     static_assert((F / R0) % SUB == 0, "pass-0 rotator reuse needs F/R0 to be a multiple of sub");
     static_assert(R0 == 16 || (R0 == 8 && F / R0 >= 64), "pass-0 writes pad(i*R0 + r) = pad(i*R0) + r");
@@ -830,6 +834,8 @@ template <int LG, bool CPLX>
 static int step_q_lg(int plan)
 {
     using G0 = Geo<LG, 0, CPLX>;
+    if constexpr (nplans(LG) > 3 && !CPLX)
+        if (plan == 3) return fstage_sperm<LG, 3, CPLX>() ? G0::F / Geo<LG, 3, CPLX>::R0 / G0::SUB : 0;
     if constexpr (nplans(LG) > 2)
         if (plan == 2) return fstage_sperm<LG, 2, CPLX>() ? G0::F / Geo<LG, 2, CPLX>::R0 / G0::SUB : 0;
     if constexpr (nplans(LG) > 1)
@@ -1074,7 +1080,8 @@ void fx_fstage_kernel(FStageArgs args)
         for (int r = 0; r < 8; r++) xwl[r] = twc(xl_wk, r);
     }
     // FP8: the inverse scales of this signal's output channels, loaded once
-    // where the register budget allows (three waves per SIMD)
+    // where the register budget allows (three waves per SIMD; the radix-32
+    // 16384-point kernel, at two, spills with them)
     constexpr bool SREG = PREC == FX_PREC_F8 && fstage_wpe<LG, V, PREC, CPLX>() == 3 && !CPLX;
     [[maybe_unused]] float sreg[SREG ? NOUT : 1];
     if constexpr (SREG) {
@@ -1086,10 +1093,11 @@ void fx_fstage_kernel(FStageArgs args)
     }
 
     // Twiddle factors whose exponents depend only on the thread, loaded once
-    // where the register budget allows (three waves per SIMD, and with four at
-    // F = 4096 and, except FP8, F = 16384: 103-127 registers, 1-5% fewer VALU
-    // instructions; the FP8 16384-point kernels spill with them): the third
-    // pass's of a four-pass Stockham plan and the last pass's.
+    // where the register budget allows (three waves per SIMD, and at F = 4096
+    // and, except FP8, F = 16384: 103-127 registers, 1-5% fewer VALU
+    // instructions; the FP8 16384-point kernels spill with them, the FP16
+    // radix-32 one 36 bytes, and measured 3% faster with them than without):
+    // the third pass's of a four-pass Stockham plan and the last pass's.
     constexpr bool TWREG = (fstage_wpe<LG, V, PREC, CPLX>() == 3 || LG == 12 || (LG == 14 && PREC != FX_PREC_F8)) &&
                            !XL && !DIF && !CPLX;
     constexpr int PER2 = (F / R2) / NT > 0 ? (F / R2) / NT : 1;
@@ -1107,9 +1115,9 @@ void fx_fstage_kernel(FStageArgs args)
 
     // The FFT loop is unrolled (G <= 4) or, for FP8 (G = 8, where eight copies
     // measured slower: I-cache), runs two FFTs per trip where the register
-    // budget allows (three waves per SIMD): no register moves of the output
-    // chain and the prefetch between the FFTs of one trip.
-    constexpr int UNR = G <= 4 ? G : (fstage_wpe<LG, V, PREC, CPLX>() == 3 && !CPLX) ? 2 : 1;
+    // budget allows (three waves per SIMD or fewer): no register moves of the
+    // output chain and the prefetch between the FFTs of one trip.
+    constexpr int UNR = G <= 4 ? G : (fstage_wpe<LG, V, PREC, CPLX>() <= 3 && !CPLX) ? 2 : 1;
 #pragma unroll UNR
     for (int gi = 0; gi < G; gi++) {
         // Redefine tid for the compiler on every FFT: the LDS addresses derived
@@ -1926,6 +1934,8 @@ static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
             if (a.plan == 1) return launch_lgv<LG, 1, true>(a, prec, s);
         return launch_lgv<LG, 0, true>(a, prec, s);
     }
+    if constexpr (nplans(LG) > 3)
+        if (a.plan == 3) return launch_lgv<LG, 3, false>(a, prec, s);
     if constexpr (nplans(LG) > 2)
         if (a.plan == 2) return launch_lgv<LG, 2, false>(a, prec, s);
     if constexpr (nplans(LG) > 1)
@@ -1933,12 +1943,13 @@ static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
     return launch_lgv<LG, 0, false>(a, prec, s);
 }
 
-// the 32768-point halves use the Stockham plans only
-int fstage_plans(int lg) { return lg == kEoLG ? 2 : nplans(lg); }
+// the 32768-point halves use the four-pass Stockham plans only, complex
+// input at F = 16384 (twice the channels) not the radix-32 plan
+int fstage_plans(int lg, int iscomplex) { return lg == kEoLG ? 2 : (lg == 14 && iscomplex) ? 3 : nplans(lg); }
 
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
 {
-    if (a.plan < 0 || a.plan >= fstage_plans(a.p.lg)) return hipErrorInvalidValue;
+    if (a.plan < 0 || a.plan >= fstage_plans(a.p.lg, a.p.iscomplex)) return hipErrorInvalidValue;
     switch (a.p.lg) {
     case 9:  return launch_lg<9>(a, prec, s);
     case 10: return launch_lg<10>(a, prec, s);

@@ -30,7 +30,7 @@ struct FStageArgs {
     float* ltap;                  // optional decoded 2-bit levels [nffts][nant][2][F] of the
                                   // valid FFTs (real input; only with tap)
     void* stage;                  // F = 32768: row staging, fx_fstage_stage_bytes(lg, prec, nant, ngroups) bytes
-    int plan;                     // radix plan, 0 .. fstage_plans(lg) - 1
+    int plan;                     // radix plan, 0 .. fstage_plans(lg, iscomplex) - 1
     int tile_ant;                 // antennas per work tile (set by launch_fstage)
 };
 
@@ -78,7 +78,7 @@ hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s);
 size_t fx_fstage_stage_bytes(int lg, int prec, int nant, int ngroups);
 bool fstage_supported(int lg);
 bool fstage_supported_complex(int lg);   // complex 8-bit input, F = nchan
-int fstage_plans(int lg);
+int fstage_plans(int lg, int iscomplex);
 // Batched forward C2C FFT of length 2^lg (natural order in/out, unnormalised).
 hipError_t launch_fft_c2c(const float2* in, float2* out, const float2* tw, int lg, int batch, hipStream_t s);
 

@@ -35,7 +35,7 @@ def _rel(a, b, axis=-1):
     return np.linalg.norm(a - b, axis=axis) / np.maximum(np.linalg.norm(b, axis=axis), 1e-30)
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
 @pytest.mark.parametrize("nchan", [512, 1024, 2048, 4096, 8192, 16384])
 def test_complex_fstage_fp32(nchan, plan):
     nant, nf = 3, 4

@@ -79,7 +79,7 @@ def test_delay_table_bit_exact(nchan):
     assert 0 < ref["valid"].sum() < nf * nant          # the case exercises invalid antennas
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
 @pytest.mark.parametrize("nchan", [256, 512, 1024, 2048, 4096, 8192, 16384])
 def test_fstage_fp32_matches_oracle(nchan, plan):
     nant, nf = 3, 6
@@ -315,7 +315,7 @@ def test_radix_plans_agree_in_low_precision(nchan, prec):
 
 
 @pytest.mark.parametrize("prec", ["f16", "f8"])
-@pytest.mark.parametrize("plan", [0, 1, 2])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
 @pytest.mark.parametrize("nchan,nf", [(512, 32), (1024, 32), (2048, 24), (4096, 16), (8192, 16)])
 def test_vis_lowprec_every_plan_vs_fp64_oracle(nchan, nf, plan, prec):
     # every radix plan of the FP16 / FP8 F-stage against the FP64 oracle: the
