@@ -242,7 +242,7 @@ int fx_get_geometry(const fx_context *ctx, int *out11);
  * FX_OPT_XTILE   X-engine workgroup tile, a square of 1, 2 or 4 blocks of
  *                32x32 signals per panel; 0 selects by the signal count
  * FX_OPT_FPLAN   F-stage radix plan, 0 .. FX_OPT_NPLANS - 1
- * FX_OPT_NPLANS  (read only) number of radix plans at this FFT length
+ * FX_OPT_NPLANS  (read only) number of radix plans at this FFT length and input
  * FX_OPT_XBATCH  subints per X-engine launch recommended to callers that
  *                batch fx_fstage calls before one fx_xstage (default 1)
  * FX_OPT_GRAPH   1 (default): fx_subint / fx_process launch recorded graphs;
