@@ -1521,7 +1521,7 @@ size_t fx_fstage_stage_bytes(int lg, int prec, int nant, int ngroups)
 }
 
 template <int V, int PREC, bool TAP>
-__global__ __launch_bounds__((Geo<kEoLG - 1, V>::NT)) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__((Geo<kEoLG - 1, V>::NT)) __attribute__((amdgpu_waves_per_eu(Geo<kEoLG - 1, V>::WPE)))
 void fx_fstage_eo_kernel(FStageArgs args)
 {
     using Gh = Geo<kEoLG - 1, V>;                               // geometry of one half
@@ -1530,8 +1530,8 @@ void fx_fstage_eo_kernel(FStageArgs args)
     constexpr int STRIDE = 1 << ((LG - 1) / 2), SUB = 2 * STRIDE, NSTEP = F / SUB, NFSTEP = NCH / STRIDE;
     constexpr int NTAB = SUB + NSTEP + STRIDE + NFSTEP;
     constexpr int R0 = Gh::R0, R1 = Gh::R1, R2 = Gh::R2, RL = Gh::RL;
-    constexpr int T0 = H / R0, TL = H / RL, PERL = TL / NT, PL = H / RL;
-    static_assert(Gh::NP == 4 && T0 == NT, "pass 0: one butterfly per thread");
+    constexpr int T0 = H / R0, TL = H / RL, PERL = TL / NT, PL = H / RL, PER0 = T0 / NT;
+    static_assert(Gh::KIND == 0 && PER0 * NT == T0, "Stockham plans, whole pass-0 butterflies per thread");
     static_assert((2 * T0) % SUB == 0, "pass-0 step rotator is a base plus r times a constant");
     static_assert(PERL * NT == PL && RL * PL == H, "last pass covers every bin of a half");
     constexpr int G = GroupSize<PREC>::G;
@@ -1578,11 +1578,11 @@ void fx_fstage_eo_kernel(FStageArgs args)
     // registers before this FFT's output stores are issued (loads issued
     // after them queue behind them); the delay entry (offset, valid) one FFT
     // earlier still.
-    static_assert(NTAB <= NT, "one table entry per thread");
+    constexpr int TPT = (NTAB + NT - 1) / NT;
     constexpr int WPT = (NWIN + NT - 1) / NT;
     struct Prefetch {
         int valid, offset;
-        float2 t;
+        float2 t[TPT];
         uint32_t w[WPT];
     };
     auto dload = [&](int gi) { return load_delay_ov(args.delays + (long long)(grp * G + gi) * p.nant + ant); };
@@ -1594,7 +1594,11 @@ void fx_fstage_eo_kernel(FStageArgs args)
         const int bb = (f.offset >> 1) & ~3;
         const int nd = ((((f.offset + F - 1) >> 1) - bb) >> 2) + 1;
         const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src + bb);
-        f.t = threadIdx.x < NTAB ? tb[threadIdx.x] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int t = 0; t < TPT; t++) {
+            const int e = threadIdx.x + t * NT;
+            f.t[t] = e < NTAB ? tb[e] : make_float2(0.0f, 0.0f);
+        }
 #pragma unroll
         for (int w = 0; w < WPT; w++) {
             const int k = threadIdx.x + w * NT;
@@ -1625,7 +1629,11 @@ void fx_fstage_eo_kernel(FStageArgs args)
             const int o = cur.offset;
             const int bb = (o >> 1) & ~3;
             const int nd = ((((o + F - 1) >> 1) - bb) >> 2) + 1;
-            if (tid < NTAB) tab[tid] = cur.t;
+#pragma unroll
+            for (int t = 0; t < TPT; t++) {
+                const int e = tid + t * NT;
+                if (e < NTAB) tab[e] = cur.t[t];
+            }
 #pragma unroll
             for (int w = 0; w < WPT; w++) {
                 const int k = tid + w * NT;
@@ -1638,8 +1646,9 @@ void fx_fstage_eo_kernel(FStageArgs args)
                 asm volatile("" : "+v"(tid));
                 __builtin_assume(tid >= 0 && tid < NT);
                 // ---- pass 0 of half h: samples n = 2 (i + r T0) + h, one time slot
-                {
-                    const int i = tid;
+#pragma unroll
+                for (int m0 = 0; m0 < PER0; m0++) {
+                    const int i = tid + m0 * NT;
                     const int n0 = 2 * i + h;
                     const float2 rs = rsub[n0 & (SUB - 1)];
                     const float2* sp = rstep + n0 / SUB;
@@ -1660,7 +1669,7 @@ void fx_fstage_eo_kernel(FStageArgs args)
                 }
                 __syncthreads();
                 mid_pass<Gh, R1, R0, true, 2>(buf, tw1, args.tw, tid);
-                mid_pass<Gh, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
+                if constexpr (Gh::NP == 4) mid_pass<Gh, R2, R0 * R1, false, 2>(buf, tw1, args.tw, tid);
                 // ---- last pass of half h, every bin, into registers.  One
                 // table entry per butterfly, all loaded ahead: W^(2i) for the
                 // first half, W^i for the second (its square is W^(2i)); the
@@ -1676,27 +1685,37 @@ void fx_fstage_eo_kernel(FStageArgs args)
 #pragma unroll
                     for (int r = 0; r < RL; r++) x[r] = rd[Gh::pad(r * TL)];
                     {
-                        float2 w[RL];
-                        w[1] = h == 0 ? lw[m] : cmul(lw[m], lw[m]);
+                        // the powers w^(2^b) are kept; w^r is the product of
+                        // those of r's set bits (at most log2(RL) - 1 products)
+                        constexpr int NB = clog2(RL);
+                        float2 pw[NB];
+                        pw[0] = h == 0 ? lw[m] : cmul(lw[m], lw[m]);
 #pragma unroll
-                        for (int r = 2; r < RL; r++) {
-                            const int hb = 1 << clog2(r);                  // largest power of 2 <= r
-                            w[r] = r == hb ? cmul(w[r / 2], w[r / 2]) : cmul(w[hb], w[r - hb]);
+                        for (int b = 1; b < NB; b++) pw[b] = cmul(pw[b - 1], pw[b - 1]);
+#pragma unroll
+                        for (int r = 1; r < RL; r++) {
+                            float2 w = pw[clog2(r)];
+#pragma unroll
+                            for (int b = 0; b < clog2(r); b++)
+                                if ((r >> b) & 1) w = cmul(w, pw[b]);
+                            x[r] = cmul(x[r], w);
                         }
-#pragma unroll
-                        for (int r = 1; r < RL; r++) x[r] = cmul(x[r], w[r]);
                     }
                     dft<RL>(x);
                     if (h == 0) {
 #pragma unroll
                         for (int r = 0; r < RL; r++) ev[m][r] = x[r];
                     } else {
-                        // W^c = W^i * exp(-2 pi i r PL / F); PL / F = 1/8
+                        // W^c = W^i * exp(-2 pi i r PL / F), PL / F = 1 / (2 RL): a
+                        // multiple of 1/32 turn for RL <= 16, otherwise the table
+                        // entry W^(r PL), the same in every lane (a scalar load)
                         const float2 wi = lw[m];
 #pragma unroll
                         for (int r = 0; r < RL; r++) {
                             const int c = i + r * PL;
-                            const float2 ov = twc(cmul(x[r], wi), r * (32 * PL / F));
+                            float2 ov;
+                            if constexpr ((32 * PL) % F == 0) ov = twc(cmul(x[r], wi), r * (32 * PL / F));
+                            else ov = cmul(cmul(x[r], wi), load_const(args.tw + r * PL));
                             const float2 X = cadd(ev[m][r], ov);
                             const float2 fr = cmul(frsub[c & (STRIDE - 1)], frstep[c / STRIDE]);
                             ev[m][r] = cmul(fr, X);
@@ -1943,9 +1962,10 @@ static hipError_t launch_lg(const FStageArgs& a, int prec, hipStream_t s)
     return launch_lgv<LG, 0, false>(a, prec, s);
 }
 
-// the 32768-point halves use the four-pass Stockham plans only, complex
-// input at F = 16384 (twice the channels) not the radix-32 plan
-int fstage_plans(int lg, int iscomplex) { return lg == kEoLG ? 2 : (lg == 14 && iscomplex) ? 3 : nplans(lg); }
+// the 32768-point halves use the Stockham plans of F = 16384 (plans 0, 1 and,
+// as plan 2, the radix-32 one); complex input at F = 16384 (twice the
+// channels) not the radix-32 plan
+int fstage_plans(int lg, int iscomplex) { return lg == kEoLG ? 3 : (lg == 14 && iscomplex) ? 3 : nplans(lg); }
 
 hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
 {
@@ -1959,7 +1979,7 @@ hipError_t launch_fstage(const FStageArgs& a, int prec, hipStream_t s)
     case 14: return launch_lg<14>(a, prec, s);
     case 15:
         if (a.p.iscomplex) return hipErrorInvalidValue;
-        return a.plan == 1 ? launch_eo<1>(a, prec, s) : launch_eo<0>(a, prec, s);
+        return a.plan == 2 ? launch_eo<3>(a, prec, s) : a.plan == 1 ? launch_eo<1>(a, prec, s) : launch_eo<0>(a, prec, s);
     default: return hipErrorInvalidValue;
     }
 }
