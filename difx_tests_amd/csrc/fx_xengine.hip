@@ -887,10 +887,12 @@ static hipError_t launch_x_prec(const XArgs& a, hipStream_t s)
                 break;
             }
             if (a.p.nblk == 1) {                          // 17 to 32 signals: three 16x16 quarters
+                // (two chunks per step in large grids: 16 x 2048 X-engine
+                // 0.654 -> 0.645 ms per 16 subints, three A/B pairs)
                 if ((long long)a.p.nchan * npairs <= kSmallGrid)
                     fx_xengine_kernel<PREC, 1, DEEP, WAVES, false, false, true><<<grid, block, 0, s>>>(a);
                 else
-                    fx_xengine_kernel<PREC, 1, 1, WAVES, false, false, true><<<grid, block, 0, s>>>(a);
+                    fx_xengine_kernel<PREC, 1, 2, WAVES, false, false, true><<<grid, block, 0, s>>>(a);
                 break;
             }
         }
