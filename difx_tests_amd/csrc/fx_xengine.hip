@@ -939,26 +939,31 @@ hipError_t launch_xengine(const XArgs& a, int prec, hipStream_t s)
 // X-engine launch reduces.  One thread per (antenna pair j <= k, run of 32
 // slots); the per-pair sums of the runs meet in counts[j*nant + k] through
 // atomic adds.
-__global__ void fx_counts_kernel(int nant, int nslots, const uint8_t* __restrict__ v, int* __restrict__ counts)
+__global__ void fx_counts_kernel(int nant, int nslots, int chunk_len, const uint8_t* __restrict__ v,
+                                 int* __restrict__ counts)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int pairs = nant * nant;
     const int chunk = (int)(t / pairs), jk = (int)(t - (long long)chunk * pairs);
     const int j = jk / nant, k = jk - j * nant;
-    const int f0 = chunk * 32;
+    const long long f0 = (long long)chunk * chunk_len;
     if (f0 >= nslots || j > k) return;
-    const int f1 = f0 + 32 < nslots ? f0 + 32 : nslots;
+    const int f1 = f0 + chunk_len < nslots ? (int)f0 + chunk_len : nslots;
     int n = 0;
 #pragma unroll 8
-    for (int f = f0; f < f1; f++) n += v[(long long)f * nant + j] & v[(long long)f * nant + k];
+    for (int f = (int)f0; f < f1; f++) n += v[(long long)f * nant + j] & v[(long long)f * nant + k];
     if (n) atomicAdd(counts + jk, n);
 }
 
 hipError_t launch_counts(int nant, int nslots, const uint8_t* valid_slots, int* counts, hipStream_t s)
 {
-    const long long n = (long long)nant * nant * ((nslots + 31) / 32);
+    // at most 64 chunks (one atomic each) per counter: the atomics on one
+    // address serialise (4 antennas x 40000 slots in chunks of 32 took 48 us)
+    const int per = (nslots + 63) / 64;
+    const int chunk_len = per <= 32 ? 32 : (per + 31) / 32 * 32;
+    const long long n = (long long)nant * nant * ((nslots + chunk_len - 1) / chunk_len);
     if (n <= 0) return hipSuccess;
-    fx_counts_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nant, nslots, valid_slots, counts);
+    fx_counts_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nant, nslots, chunk_len, valid_slots, counts);
     return hipGetLastError();
 }
 
